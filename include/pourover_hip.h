@@ -1,0 +1,249 @@
+/*
+ * pourover_hip.h -- C ABI of the MI355X (gfx950) D3Q19 lattice-Boltzmann engine.
+ *
+ * The library (libpourover_hip.so) owns no field memory: the caller allocates
+ * every device buffer (PyTorch-ROCm tensors in the Python host layer) and
+ * passes raw device pointers, which the library borrows and never frees.  A
+ * context owns only its descriptor and a small scratch area (reduction
+ * partials) allocated with hipMalloc in pol_create and released in
+ * pol_destroy.
+ *
+ * Every entry point returns an int status (POL_OK == 0).  On failure the
+ * context keeps a message readable with pol_last_error().  Work is enqueued
+ * asynchronously on the hipStream_t passed as `stream` (NULL = legacy
+ * default stream); only entry points that hand results back to host memory
+ * (pol_reduce, pol_macroscopic / pol_step_ex / pol_step_from with a
+ * reduction, pol_adjoint_collect) synchronise that stream.
+ *
+ * The reference (latteine1217/pour-over-coffee-lbm) is Python + Taichi and
+ * has no C ABI; each entry point below names the reference interface whose
+ * job it takes over.
+ *
+ * Population layout in HBM: f[q][z][y][x], q in the D3Q19 order of
+ * config/core.py:36-38, x fastest, two ping-pong buffers.  Buffers hold
+ * post-collision populations; one step gathers (pull streaming) from one
+ * buffer and writes the other.  Every storage format (F32, F16, BF16, F64)
+ * holds the scaled deviation D_q = 36 (f_q - w_q) from the lattice weight
+ * (36 w_q = 12, 2, 1 are exact, so no rounded weight enters the arithmetic).
+ */
+#ifndef POUROVER_HIP_H
+#define POUROVER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POL_ABI_VERSION 1
+#define POL_Q 19
+#define POL_NRED 12 /* length of the reduction vector, see POL_RED_* */
+
+/* status codes; mapped on the Python side onto the reference's BackendError
+ * hierarchy (src/core/backends/compute_backends.py:35-65) */
+enum {
+  POL_OK = 0,
+  POL_ERR_INVALID = 1,     /* bad argument / descriptor -> ComputeExecutionError */
+  POL_ERR_HIP = 2,         /* HIP runtime error        -> ComputeExecutionError */
+  POL_ERR_ALLOC = 3,       /* scratch allocation       -> MemoryAllocationError */
+  POL_ERR_NO_DEVICE = 4,   /* no usable GPU            -> PLATFORM_UNAVAILABLE  */
+  POL_ERR_STATE = 5,       /* call order (e.g. not bound) */
+  POL_ERR_UNSUPPORTED = 6
+};
+
+/* population storage formats */
+enum { POL_F32 = 0, POL_F16 = 1, POL_BF16 = 2, POL_F64 = 3 };
+
+/* pol_desc.options */
+#define POL_PERIODIC_X 0x1u
+#define POL_PERIODIC_Y 0x2u
+#define POL_PERIODIC_Z 0x4u
+#define POL_USE_FLAGS 0x8u      /* cell_flags buffer drives walls / cell kinds */
+#define POL_LES 0x10u           /* Smagorinsky eddy viscosity */
+#define POL_FORCE_FIELD 0x20u   /* add per-cell force density buffer */
+#define POL_PERM_FIELD 0x40u    /* per-cell permeability buffer (else darcy_K) */
+#define POL_GRAVITY 0x80u       /* constant body-force density gravity[3] */
+
+/* cell flag word (uint32 per cell) */
+#define POL_LINK_MASK 0x7FFFFu  /* bit q: source x - e_q of link q is a wall -> halfway bounce-back */
+#define POL_KIND_SHIFT 24
+#define POL_KIND_MASK 0x7u
+#define POL_KIND_FLUID 0u
+#define POL_KIND_SOLID 1u
+#define POL_KIND_INLET 2u       /* equilibrium velocity inlet (inlet_rho, inlet_u) */
+#define POL_KIND_OUTLET 3u      /* pressure outlet: f* = feq(outlet_rho, u) */
+#define POL_FLAG_POROUS 0x08000000u /* Darcy-Forchheimer resistance, LES off */
+#define POL_FLAG_NO_LES 0x10000000u
+
+/* pol_step_ex / pol_macroscopic `what` bits */
+#define POL_WANT_MACRO 0x1u   /* write rho/u buffers */
+#define POL_WANT_REDUCE 0x2u  /* FP64 reductions into red_out[POL_NRED] */
+
+/* reduction vector slots (FP64) */
+#define POL_RED_MASS 0      /* sum rho over non-solid cells */
+#define POL_RED_MOMX 1      /* sum rho*u */
+#define POL_RED_MOMY 2
+#define POL_RED_MOMZ 3
+#define POL_RED_KE 4        /* sum rho*|u|^2/2 */
+#define POL_RED_UMAX 5      /* max |u| */
+#define POL_RED_NONFINITE 6 /* count of cells with non-finite rho or u */
+#define POL_RED_OUTFLUX 7   /* sum over outlet cells of -rho*u_z */
+#define POL_RED_CELLS 8     /* number of non-solid cells */
+#define POL_RED_RHOMIN 9
+#define POL_RED_RHOMAX 10
+#define POL_RED_INFLUX 11   /* sum over inlet cells of -rho*u_z */
+
+typedef struct pol_ctx pol_ctx;
+
+typedef struct pol_desc {
+  int32_t nx, ny, nz;
+  int32_t storage;     /* POL_F32 / POL_F16 / POL_BF16 / POL_F64 */
+  uint32_t options;    /* POL_PERIODIC_* | POL_USE_FLAGS | ... */
+  int32_t device;      /* HIP device ordinal the buffers live on */
+  double tau;          /* molecular relaxation time (nu = (tau - 1/2)/3) */
+  double cs_smag;      /* Smagorinsky constant C_s (filter width 1 lattice unit) */
+  double nu_sgs_max;   /* clamp on the eddy viscosity */
+  double gravity[3];   /* constant body-force density (POL_GRAVITY) */
+  double darcy_K;      /* uniform permeability in porous cells */
+  double forch_CF;     /* Forchheimer coefficient C_F */
+  double inlet_rho;
+  double inlet_u[3];
+  double outlet_rho;
+} pol_desc;
+
+/* Device buffers.  "real" = double for POL_F64 storage, float otherwise. */
+typedef struct pol_buffers {
+  void *f[2];           /* [19][nz][ny][nx] storage type, ping-pong */
+  uint32_t *cell_flags; /* [nz][ny][nx] (POL_USE_FLAGS) */
+  void *force;          /* real [3][cells] force density (POL_FORCE_FIELD) */
+  void *perm;           /* real [cells] permeability (POL_PERM_FIELD) */
+  void *rho;            /* real [cells] macroscopic output, may be NULL */
+  void *u;              /* real [3][cells] macroscopic output, may be NULL */
+} pol_buffers;
+
+/* ---- library ------------------------------------------------------------ */
+/* ABI version (POL_ABI_VERSION). */
+int pol_version(void);
+/* Static message for a status code. */
+const char *pol_status_string(int status);
+/* Number of HIP devices.  Replaces PlatformDetector platform probing
+ * (src/core/backends/compute_backends.py:121-329). */
+int pol_device_count(int *count);
+
+/* ---- context lifecycle --------------------------------------------------- */
+/* Create a solver context.  Replaces ComputeBackend.initialize_backend
+ * (compute_backends.py:429-441) and the solver construction of
+ * UnifiedLBMSolver (src/core/lbm_unified.py:44-66). */
+int pol_create(pol_ctx **out, const pol_desc *desc);
+/* Release the context and its scratch.  Replaces cleanup_backend
+ * (compute_backends.py:443-452). */
+int pol_destroy(pol_ctx *ctx);
+/* Last error message of this context ("" if none). */
+const char *pol_last_error(const pol_ctx *ctx);
+/* Attach caller-owned device buffers.  Replaces the memory-adapter attribute
+ * contract f/f_new/rho/u/solid (src/core/backends/cpu_backend.py:207-212). */
+int pol_bind(pol_ctx *ctx, const pol_buffers *bufs);
+/* Index (0/1) of the buffer holding the current state. */
+int pol_current(const pol_ctx *ctx, int *index);
+/* Select which buffer holds the current state (after a host-side restore). */
+int pol_set_current(pol_ctx *ctx, int index);
+
+/* ---- hot path ------------------------------------------------------------ */
+/* f[current] = feq(rho, u) per cell (rho NULL -> 1, u NULL -> 0), computed in
+ * the real type.  Replaces LBMSolver.init_fields
+ * (src/core/legacy/lbm_solver.py:1067-1112). */
+int pol_init_equilibrium(pol_ctx *ctx, const void *rho, const void *u, void *stream);
+/* nsteps fused stream-collide passes.  Replaces
+ * ComputeBackend.execute_collision_streaming (compute_backends.py:367-378,
+ * cpu_backend.py:192-237) and LBMSolver._collision_streaming_step +
+ * swap_fields (legacy/lbm_solver.py:460-487,630-654). */
+int pol_step(pol_ctx *ctx, int nsteps, void *stream);
+/* As pol_step; the last step also writes rho/u and/or fills red_out (FP64,
+ * POL_NRED values, moments of the state that step collided). */
+int pol_step_ex(pol_ctx *ctx, int nsteps, uint32_t what, double *red_out, void *stream);
+/* One step from an explicit buffer `fin` into `fout` (same layout/storage as
+ * the bound f); the context's current-buffer index is left unchanged.  Used
+ * to record forward states for the adjoint's checkpoint recomputation. */
+int pol_step_from(pol_ctx *ctx, const void *fin, void *fout, uint32_t what, double *red_out, void *stream);
+/* Moments of the current state (the gather the next step would collide):
+ * rho/u buffers and/or FP64 reductions.  Replaces
+ * compute_macroscopic_quantities (compute_backends.py:393-404,
+ * legacy/lbm_solver.py:488-535) and the statistics reductions of
+ * visualizer.py:120-167 / numerical_stability.py:52-110. */
+int pol_macroscopic(pol_ctx *ctx, uint32_t what, double *red_out, void *stream);
+/* FP64 reductions of the current state; synchronous.  = pol_macroscopic
+ * with POL_WANT_REDUCE. */
+int pol_reduce(pol_ctx *ctx, double *red_out, void *stream);
+
+/* ---- Lagrangian coffee particles (src/physics/coffee_particles.py) ------- */
+typedef struct pol_particle_desc {
+  int32_t count;         /* number of particle slots */
+  int32_t reserved;
+  double rho_fluid;      /* fluid density in drag law */
+  double nu_fluid;       /* kinematic viscosity (mu = rho_fluid * nu_fluid) */
+  double drag_max_per_mass; /* |drag| <= this * mass */
+  double relax_alpha;    /* under-relaxation factor */
+  double gravity[3];     /* acceleration on particles */
+  double dt;             /* particle time step (lattice units) */
+  double max_disp;       /* displacement clamp per step */
+  double wall_damping;   /* velocity factor when a move is rejected */
+} pol_particle_desc;
+
+/* All vectors SoA [3][count], in the real type of the context (double for
+ * POL_F64 storage, float otherwise); active is int32. */
+typedef struct pol_particle_buffers {
+  void *pos, *vel, *radius, *mass;
+  void *active;
+  void *drag_new, *drag_old, *drag; /* [3][count] */
+  void *u_fluid;  /* [3][count] interpolated fluid velocity (diagnostic) */
+  void *re_p, *cd; /* [count] diagnostics */
+  void *force;    /* [3][cells] reaction-force field (= pol_buffers.force) */
+  const void *u_field; /* [3][cells] fluid velocity field read by the gather */
+} pol_particle_buffers;
+
+/* Attach a particle system.  Replaces CoffeeParticleSystem field set-up
+ * (coffee_particles.py:30-64). */
+int pol_particles_bind(pol_ctx *ctx, const pol_particle_desc *d, const pol_particle_buffers *b);
+/* Interpolate, drag, under-relax and spread the reaction force.  Replaces
+ * compute_two_way_coupling_forces + apply_under_relaxation
+ * (coffee_particles.py:1107-1154,1200-1212). */
+int pol_particles_couple(pol_ctx *ctx, void *stream);
+/* Clear this step's reaction-force footprint and advance the particles.
+ * Replaces clear_reaction_forces + update_particle_physics
+ * (coffee_particles.py:1101-1105,641-720). */
+int pol_particles_advance(pol_ctx *ctx, void *stream);
+/* Particle -> cell index and clamped fractions, the formula of
+ * coffee_particles.py:1159-1174: i = (int)max(0, min(n-2, p)),
+ * frac = clamp(p - i, 0, 1).  pos SoA [3][n]; idx [3][n]; frac [3][n]. */
+int pol_particle_index(const float *pos, int32_t n, int32_t nx, int32_t ny, int32_t nz,
+                       int32_t *idx, float *frac, void *stream);
+
+/* ---- adjoint (no counterpart in the reference) ---------------------------- */
+typedef struct pol_adjoint_buffers {
+  void *lam[2];        /* adjoint populations [19][cells], real type */
+  double *grad_perm;   /* [cells] accumulated dJ/dK (FP64), may be NULL */
+  uint32_t *rev_flags; /* [cells] reverse-gather word: bit p = the cell
+                          x + e_p gathers population p of x normally;
+                          bit 31 = clamped readers possible (open face) */
+} pol_adjoint_buffers;
+/* Attach adjoint buffers (lam[0] is the current adjoint state).  F32 and F64
+ * storage only. */
+int pol_adjoint_bind(pol_ctx *ctx, const pol_adjoint_buffers *b);
+/* One reverse step: given the forward state `f_state` (the buffer the forward
+ * step gathered from, same layout/storage as f) and the current adjoint, write
+ * the adjoint of f_state.  With objective != 0 the step's outlet mass flux
+ * (sum over outlet cells of -rho*u_z, POL_RED_OUTFLUX) is added to J.
+ * dJ/dK accumulates in grad_perm, dJ/d(inlet_u) and J in the context (FP64).
+ * Steps must be issued in reverse time order. */
+int pol_adjoint_step(pol_ctx *ctx, const void *f_state, int objective, void *stream);
+/* Zero the adjoint state, grad_perm and the inlet-velocity gradient. */
+int pol_adjoint_reset(pol_ctx *ctx, void *stream);
+/* Copy the FP64 accumulators {dJ/du_in x, y, z, J} (4 doubles) to host;
+ * synchronous. */
+int pol_adjoint_collect(pol_ctx *ctx, double *out4, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POUROVER_HIP_H */

@@ -1,0 +1,161 @@
+"""ctypes binding of libpourover_hip.so (the C ABI of include/pourover_hip.h).
+
+The library is loaded after torch so that both share torch's HIP runtime
+(libamdhip64.so.7 is matched by soname).  A missing library raises
+ImportError: there is no CPU fallback for the hot path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede the library: one HIP runtime per process)
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libpourover_hip.so"
+
+# ---- constants (mirror include/pourover_hip.h) ------------------------------
+POL_ABI_VERSION = 1
+POL_Q = 19
+POL_NRED = 12
+POL_OK, POL_ERR_INVALID, POL_ERR_HIP, POL_ERR_ALLOC, POL_ERR_NO_DEVICE, POL_ERR_STATE, POL_ERR_UNSUPPORTED = range(7)
+POL_F32, POL_F16, POL_BF16, POL_F64 = range(4)
+POL_PERIODIC_X = 0x1
+POL_PERIODIC_Y = 0x2
+POL_PERIODIC_Z = 0x4
+POL_USE_FLAGS = 0x8
+POL_LES = 0x10
+POL_FORCE_FIELD = 0x20
+POL_PERM_FIELD = 0x40
+POL_GRAVITY = 0x80
+POL_LINK_MASK = 0x7FFFF
+POL_KIND_SHIFT = 24
+POL_KIND_MASK = 0x7
+POL_KIND_FLUID, POL_KIND_SOLID, POL_KIND_INLET, POL_KIND_OUTLET = 0, 1, 2, 3
+POL_FLAG_POROUS = 0x08000000
+POL_FLAG_NO_LES = 0x10000000
+POL_RFLAG_CLAMP = 0x80000000
+POL_WANT_MACRO = 0x1
+POL_WANT_REDUCE = 0x2
+RED_SLOTS = ("mass", "mom_x", "mom_y", "mom_z", "kinetic_energy", "max_velocity", "nonfinite_cells",
+             "outlet_flux", "cells", "rho_min", "rho_max", "inlet_flux")
+
+STORAGE_CODES = {"f32": POL_F32, "f16": POL_F16, "bf16": POL_BF16, "f64": POL_F64}
+
+
+class PolDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("storage", C.c_int32), ("options", C.c_uint32), ("device", C.c_int32),
+        ("tau", C.c_double), ("cs_smag", C.c_double), ("nu_sgs_max", C.c_double),
+        ("gravity", C.c_double * 3), ("darcy_K", C.c_double), ("forch_CF", C.c_double),
+        ("inlet_rho", C.c_double), ("inlet_u", C.c_double * 3), ("outlet_rho", C.c_double),
+    ]
+
+
+class PolBuffers(C.Structure):
+    _fields_ = [
+        ("f", C.c_void_p * 2), ("cell_flags", C.c_void_p), ("force", C.c_void_p),
+        ("perm", C.c_void_p), ("rho", C.c_void_p), ("u", C.c_void_p),
+    ]
+
+
+class PolParticleDesc(C.Structure):
+    _fields_ = [
+        ("count", C.c_int32), ("reserved", C.c_int32),
+        ("rho_fluid", C.c_double), ("nu_fluid", C.c_double), ("drag_max_per_mass", C.c_double),
+        ("relax_alpha", C.c_double), ("gravity", C.c_double * 3), ("dt", C.c_double),
+        ("max_disp", C.c_double), ("wall_damping", C.c_double),
+    ]
+
+
+class PolParticleBuffers(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "pos", "vel", "radius", "mass", "active", "drag_new", "drag_old", "drag", "u_fluid", "re_p", "cd",
+        "force", "u_field")]
+
+
+class PolAdjointBuffers(C.Structure):
+    _fields_ = [("lam", C.c_void_p * 2), ("grad_perm", C.c_void_p), ("rev_flags", C.c_void_p)]
+
+
+# name -> (restype, argtypes); every symbol declared in the public header
+_P = C.c_void_p
+SIGNATURES = {
+    "pol_version": (C.c_int, []),
+    "pol_status_string": (C.c_char_p, [C.c_int]),
+    "pol_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "pol_create": (C.c_int, [C.POINTER(_P), C.POINTER(PolDesc)]),
+    "pol_destroy": (C.c_int, [_P]),
+    "pol_last_error": (C.c_char_p, [_P]),
+    "pol_bind": (C.c_int, [_P, C.POINTER(PolBuffers)]),
+    "pol_current": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "pol_set_current": (C.c_int, [_P, C.c_int]),
+    "pol_init_equilibrium": (C.c_int, [_P, _P, _P, _P]),
+    "pol_step": (C.c_int, [_P, C.c_int, _P]),
+    "pol_step_ex": (C.c_int, [_P, C.c_int, C.c_uint32, _P, _P]),
+    "pol_step_from": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P]),
+    "pol_macroscopic": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "pol_reduce": (C.c_int, [_P, _P, _P]),
+    "pol_particles_bind": (C.c_int, [_P, C.POINTER(PolParticleDesc), C.POINTER(PolParticleBuffers)]),
+    "pol_particles_couple": (C.c_int, [_P, _P]),
+    "pol_particles_advance": (C.c_int, [_P, _P]),
+    "pol_particle_index": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P]),
+    "pol_adjoint_bind": (C.c_int, [_P, C.POINTER(PolAdjointBuffers)]),
+    "pol_adjoint_step": (C.c_int, [_P, _P, C.c_int, _P]),
+    "pol_adjoint_reset": (C.c_int, [_P, _P]),
+    "pol_adjoint_collect": (C.c_int, [_P, _P, _P]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load and type the library (once).  Raises ImportError if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("POUROVER_HIP_LIB", LIB_PATH))
+    if not path.exists():
+        raise ImportError(
+            "libpourover_hip.so not found at %s: build it with __graft_entry__.build() "
+            "(there is no CPU fallback for the HIP hot path)" % path)
+    lib = C.CDLL(str(path))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.pol_version() != POL_ABI_VERSION:
+        raise ImportError("libpourover_hip.so ABI %d != %d" % (lib.pol_version(), POL_ABI_VERSION))
+    _lib = lib
+    return lib
+
+
+class PolError(RuntimeError):
+    """Non-zero status from the C ABI."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        super().__init__(message)
+
+
+def check(status: int, ctx=None, what: str = "") -> None:
+    if status == POL_OK:
+        return
+    lib = load()
+    detail = ""
+    if ctx:
+        detail = (lib.pol_last_error(ctx) or b"").decode(errors="replace")
+    base = lib.pol_status_string(status).decode()
+    raise PolError(status, "%s failed: %s%s" % (what or "pol call", base, (" - " + detail) if detail else ""))
+
+
+def stream_handle(stream=None):
+    """hipStream_t of a torch stream (default: current stream)."""
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)

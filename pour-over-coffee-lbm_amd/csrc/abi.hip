@@ -1,0 +1,396 @@
+// C ABI of libpourover_hip.so (declared in include/pourover_hip.h).
+// Host-side only: validation, context state, launch dispatch.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.h"
+
+using namespace pol;
+
+struct pol_ctx {
+  pol_desc d;
+  pol_buffers b;
+  bool bound = false;
+  int cur = 0;
+  uint32_t N = 0;
+  Grid grid{};
+  double* partial = nullptr;  // [nblk][POL_NRED]
+  double* red_dev = nullptr;  // [POL_NRED]
+  // particles
+  bool particles = false;
+  ParticleArgs pa{};
+  // adjoint
+  bool adjoint = false;
+  pol_adjoint_buffers ab{};
+  int acur = 0;
+  double* adj_partial = nullptr;  // [nblk][4]
+  double* adj_acc = nullptr;      // [4]: dJ/du_in (3), J
+  std::string err;
+};
+
+static int fail(pol_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+static int hip_fail(pol_ctx* c, hipError_t e, const char* where) {
+  return fail(c, POL_ERR_HIP, std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+}
+
+static size_t storage_bytes(int s) {
+  switch (s) {
+    case POL_F32: return 4;
+    case POL_F16: return 2;
+    case POL_BF16: return 2;
+    case POL_F64: return 8;
+  }
+  return 0;
+}
+
+extern "C" {
+
+int pol_version(void) { return POL_ABI_VERSION; }
+
+const char* pol_status_string(int status) {
+  switch (status) {
+    case POL_OK: return "ok";
+    case POL_ERR_INVALID: return "invalid argument";
+    case POL_ERR_HIP: return "HIP runtime error";
+    case POL_ERR_ALLOC: return "device allocation failed";
+    case POL_ERR_NO_DEVICE: return "no HIP device";
+    case POL_ERR_STATE: return "invalid call order";
+    case POL_ERR_UNSUPPORTED: return "unsupported configuration";
+  }
+  return "unknown status";
+}
+
+int pol_device_count(int* count) {
+  if (!count) return POL_ERR_INVALID;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return e == hipErrorNoDevice ? POL_ERR_NO_DEVICE : POL_ERR_HIP;
+  }
+  *count = n;
+  return n > 0 ? POL_OK : POL_ERR_NO_DEVICE;
+}
+
+int pol_create(pol_ctx** out, const pol_desc* desc) {
+  if (!out || !desc) return POL_ERR_INVALID;
+  *out = nullptr;
+  const pol_desc& d = *desc;
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return POL_ERR_INVALID;
+  const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
+  if (N >= (1ull << 31) || (uint64_t)d.nx * d.ny >= (1ull << 31)) return POL_ERR_INVALID;
+  if (storage_bytes(d.storage) == 0) return POL_ERR_INVALID;
+  if (!(d.tau > 0.5) || !std::isfinite(d.tau)) return POL_ERR_INVALID;
+  if ((d.options & POL_LES) && !(d.cs_smag >= 0.0)) return POL_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return POL_ERR_NO_DEVICE;
+  if (d.device < 0 || d.device >= ndev) return POL_ERR_INVALID;
+  pol_ctx* c = new (std::nothrow) pol_ctx();
+  if (!c) return POL_ERR_ALLOC;
+  c->d = d;
+  c->N = (uint32_t)N;
+  c->grid = make_grid(N);
+  hipError_t e = hipSetDevice(d.device);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->partial, (size_t)c->grid.nblk * POL_NRED * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->red_dev, POL_NRED * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->adj_partial, (size_t)c->grid.nblk * 4 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->adj_acc, 4 * sizeof(double));
+  if (e != hipSuccess) {
+    hipFree(c->partial);
+    hipFree(c->red_dev);
+    hipFree(c->adj_partial);
+    hipFree(c->adj_acc);
+    delete c;
+    return POL_ERR_ALLOC;
+  }
+  *out = c;
+  return POL_OK;
+}
+
+int pol_destroy(pol_ctx* c) {
+  if (!c) return POL_OK;
+  hipSetDevice(c->d.device);
+  hipFree(c->partial);
+  hipFree(c->red_dev);
+  hipFree(c->adj_partial);
+  hipFree(c->adj_acc);
+  delete c;
+  return POL_OK;
+}
+
+const char* pol_last_error(const pol_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pol_bind(pol_ctx* c, const pol_buffers* b) {
+  if (!c || !b) return POL_ERR_INVALID;
+  if (!b->f[0] || !b->f[1] || b->f[0] == b->f[1]) return fail(c, POL_ERR_INVALID, "pol_bind: two distinct population buffers required");
+  const uint32_t o = c->d.options;
+  if ((o & POL_USE_FLAGS) && !b->cell_flags) return fail(c, POL_ERR_INVALID, "pol_bind: POL_USE_FLAGS without cell_flags");
+  if ((o & POL_FORCE_FIELD) && !b->force) return fail(c, POL_ERR_INVALID, "pol_bind: POL_FORCE_FIELD without force buffer");
+  if ((o & POL_PERM_FIELD) && !b->perm) return fail(c, POL_ERR_INVALID, "pol_bind: POL_PERM_FIELD without perm buffer");
+  if ((o & POL_PERM_FIELD) && !(o & POL_USE_FLAGS)) return fail(c, POL_ERR_INVALID, "pol_bind: permeability needs cell flags");
+  c->b = *b;
+  if (!(o & POL_USE_FLAGS)) c->b.cell_flags = nullptr;
+  c->bound = true;
+  c->cur = 0;
+  c->err.clear();
+  return POL_OK;
+}
+
+int pol_current(const pol_ctx* c, int* index) {
+  if (!c || !index) return POL_ERR_INVALID;
+  *index = c->cur;
+  return POL_OK;
+}
+
+int pol_set_current(pol_ctx* c, int index) {
+  if (!c || (index != 0 && index != 1)) return POL_ERR_INVALID;
+  c->cur = index;
+  return POL_OK;
+}
+
+int pol_init_equilibrium(pol_ctx* c, const void* rho, const void* u, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_init_equilibrium: buffers not bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_init_equilibrium(c->d.storage, c->b.f[c->cur], rho, u, c->N, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_init_equilibrium");
+}
+
+static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_t what) {
+  StepArgs a{};
+  const pol_desc& d = c->d;
+  a.fin = fin;
+  a.fout = fout;
+  a.flags = (d.options & POL_USE_FLAGS) ? c->b.cell_flags : nullptr;
+  a.force = c->b.force;
+  a.perm = c->b.perm;
+  a.rho = c->b.rho;
+  a.u = c->b.u;
+  a.partial = c->partial;
+  a.N = c->N;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.nz = d.nz;
+  a.nxy = (uint32_t)d.nx * d.ny;
+  a.dnx = make_fastdiv(d.nx);
+  a.dny = make_fastdiv(d.ny);
+  a.per_xcd = c->grid.per_xcd;
+  a.nblk = c->grid.nblk;
+  a.options = d.options;
+  a.what = what;
+  a.tau0 = d.tau;
+  a.nu0 = (d.tau - 0.5) / 3.0;
+  a.les_c = 18.0 * std::sqrt(2.0) * d.cs_smag * d.cs_smag;
+  a.tau_max = d.tau + 3.0 * d.nu_sgs_max;
+  for (int i = 0; i < 3; ++i) a.G[i] = (d.options & POL_GRAVITY) ? d.gravity[i] : 0.0;
+  a.K = d.darcy_K;
+  a.CF = d.forch_CF;
+  a.inlet_rho = d.inlet_rho;
+  for (int i = 0; i < 3; ++i) a.inlet_u[i] = d.inlet_u[i];
+  a.outlet_rho = d.outlet_rho;
+  return a;
+}
+
+static int check_what(pol_ctx* c, uint32_t what, double* red_out, const char* fn) {
+  if ((what & POL_WANT_MACRO) && (!c->b.rho || !c->b.u))
+    return fail(c, POL_ERR_STATE, std::string(fn) + ": macro output requested but rho/u not bound");
+  if ((what & POL_WANT_REDUCE) && !red_out) return fail(c, POL_ERR_INVALID, std::string(fn) + ": red_out is NULL");
+  return POL_OK;
+}
+
+static int finish_reduce(pol_ctx* c, double* red_out, hipStream_t s, const char* fn) {
+  hipError_t e = launch_reduce_final(c->partial, c->grid.nblk, c->red_dev, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(red_out, c->red_dev, POL_NRED * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, fn);
+}
+
+int pol_step_ex(pol_ctx* c, int nsteps, uint32_t what, double* red_out, void* stream) {
+  if (!c || nsteps < 0) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_step: buffers not bound");
+  int st = check_what(c, what, red_out, "pol_step");
+  if (st) return st;
+  if (nsteps == 0) return POL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
+    const uint32_t w = (i == nsteps - 1) ? what : 0u;
+    StepArgs a = make_args(c, c->b.f[c->cur], c->b.f[1 - c->cur], w);
+    e = launch_stream_collide(c->d.storage, a, true, c->grid, s);
+    if (e == hipSuccess) c->cur = 1 - c->cur;
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "pol_step");
+  if (what & POL_WANT_REDUCE) return finish_reduce(c, red_out, s, "pol_step");
+  return POL_OK;
+}
+
+int pol_step(pol_ctx* c, int nsteps, void* stream) { return pol_step_ex(c, nsteps, 0u, nullptr, stream); }
+
+int pol_step_from(pol_ctx* c, const void* fin, void* fout, uint32_t what, double* red_out, void* stream) {
+  if (!c || !fin || !fout || fin == fout) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_step_from: buffers not bound");
+  int st = check_what(c, what, red_out, "pol_step_from");
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  StepArgs a = make_args(c, fin, fout, what);
+  if (e == hipSuccess) e = launch_stream_collide(c->d.storage, a, true, c->grid, s);
+  if (e != hipSuccess) return hip_fail(c, e, "pol_step_from");
+  if (what & POL_WANT_REDUCE) return finish_reduce(c, red_out, s, "pol_step_from");
+  return POL_OK;
+}
+
+int pol_macroscopic(pol_ctx* c, uint32_t what, double* red_out, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_macroscopic: buffers not bound");
+  int st = check_what(c, what, red_out, "pol_macroscopic");
+  if (st) return st;
+  if (!what) return POL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  StepArgs a = make_args(c, c->b.f[c->cur], nullptr, what);
+  if (e == hipSuccess) e = launch_stream_collide(c->d.storage, a, false, c->grid, s);
+  if (e != hipSuccess) return hip_fail(c, e, "pol_macroscopic");
+  if (what & POL_WANT_REDUCE) return finish_reduce(c, red_out, s, "pol_macroscopic");
+  return POL_OK;
+}
+
+int pol_reduce(pol_ctx* c, double* red_out, void* stream) {
+  return pol_macroscopic(c, POL_WANT_REDUCE, red_out, stream);
+}
+
+// ---- particles ----------------------------------------------------------
+int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particle_buffers* b) {
+  if (!c || !d || !b) return POL_ERR_INVALID;
+  if (d->count < 0) return fail(c, POL_ERR_INVALID, "pol_particles_bind: negative count");
+  if (!c->bound || !(c->d.options & POL_FORCE_FIELD))
+    return fail(c, POL_ERR_STATE, "pol_particles_bind: bind forward buffers with POL_FORCE_FIELD first");
+  if (d->count > 0 && (!b->pos || !b->vel || !b->radius || !b->mass || !b->active || !b->drag_new || !b->drag_old ||
+                       !b->drag || !b->u_fluid || !b->re_p || !b->cd || !b->force || !b->u_field))
+    return fail(c, POL_ERR_INVALID, "pol_particles_bind: missing particle buffer");
+  if (!(d->relax_alpha >= 0.0 && d->relax_alpha <= 1.0)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: alpha outside [0,1]");
+  if (!(d->dt > 0.0) || !(d->max_disp > 0.0)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: dt and max_disp must be > 0");
+  ParticleArgs& p = c->pa;
+  p = ParticleArgs{};
+  p.count = d->count;
+  p.real_is_double = c->d.storage == POL_F64;
+  p.nx = c->d.nx;
+  p.ny = c->d.ny;
+  p.nz = c->d.nz;
+  p.nxy = (uint32_t)c->d.nx * c->d.ny;
+  p.N = c->N;
+  p.flags = (c->d.options & POL_USE_FLAGS) ? c->b.cell_flags : nullptr;
+  p.pos = b->pos;
+  p.vel = b->vel;
+  p.radius = b->radius;
+  p.mass = b->mass;
+  p.active = (int32_t*)b->active;
+  p.drag_new = b->drag_new;
+  p.drag_old = b->drag_old;
+  p.drag = b->drag;
+  p.u_fluid = b->u_fluid;
+  p.re_p = b->re_p;
+  p.cd = b->cd;
+  p.force = b->force;
+  p.u_field = b->u_field;
+  p.rho_fluid = d->rho_fluid;
+  p.nu_fluid = d->nu_fluid;
+  p.drag_max_per_mass = d->drag_max_per_mass;
+  p.alpha = d->relax_alpha;
+  for (int i = 0; i < 3; ++i) p.g[i] = d->gravity[i];
+  p.dt = d->dt;
+  p.max_disp = d->max_disp;
+  p.wall_damping = d->wall_damping;
+  c->particles = true;
+  return POL_OK;
+}
+
+int pol_particles_couple(pol_ctx* c, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->particles) return fail(c, POL_ERR_STATE, "pol_particles_couple: no particles bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_particles_couple(c->pa, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_couple");
+}
+
+int pol_particles_advance(pol_ctx* c, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->particles) return fail(c, POL_ERR_STATE, "pol_particles_advance: no particles bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_particles_advance(c->pa, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_advance");
+}
+
+int pol_particle_index(const float* pos, int32_t n, int32_t nx, int32_t ny, int32_t nz, int32_t* idx, float* frac,
+                       void* stream) {
+  if (n < 0 || nx < 2 || ny < 2 || nz < 2) return POL_ERR_INVALID;
+  if (n > 0 && (!pos || !idx || !frac)) return POL_ERR_INVALID;
+  hipError_t e = launch_particle_index(pos, n, nx, ny, nz, idx, frac, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : POL_ERR_HIP;
+}
+
+// ---- adjoint ---------------------------------------------------------------
+int pol_adjoint_bind(pol_ctx* c, const pol_adjoint_buffers* b) {
+  if (!c || !b) return POL_ERR_INVALID;
+  if (c->d.storage != POL_F32 && c->d.storage != POL_F64)
+    return fail(c, POL_ERR_UNSUPPORTED, "pol_adjoint_bind: adjoint supports F32 and F64 storage");
+  if (!(c->d.options & POL_USE_FLAGS) || !c->bound)
+    return fail(c, POL_ERR_STATE, "pol_adjoint_bind: bind forward buffers with cell flags first");
+  if (!b->lam[0] || !b->lam[1] || b->lam[0] == b->lam[1] || !b->rev_flags)
+    return fail(c, POL_ERR_INVALID, "pol_adjoint_bind: two adjoint buffers and rev_flags required");
+  c->ab = *b;
+  c->acur = 0;
+  c->adjoint = true;
+  return POL_OK;
+}
+
+int pol_adjoint_reset(pol_ctx* c, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->adjoint) return fail(c, POL_ERR_STATE, "pol_adjoint_reset: adjoint not bound");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t rb = c->d.storage == POL_F64 ? 8 : 4;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[0], 0, (size_t)POL_Q * c->N * rb, s);
+  if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[1], 0, (size_t)POL_Q * c->N * rb, s);
+  if (e == hipSuccess && c->ab.grad_perm) e = hipMemsetAsync(c->ab.grad_perm, 0, (size_t)c->N * sizeof(double), s);
+  if (e == hipSuccess) e = hipMemsetAsync(c->adj_acc, 0, 4 * sizeof(double), s);
+  c->acur = 0;
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_adjoint_reset");
+}
+
+int pol_adjoint_step(pol_ctx* c, const void* f_state, int objective, void* stream) {
+  if (!c || !f_state) return POL_ERR_INVALID;
+  if (!c->adjoint) return fail(c, POL_ERR_STATE, "pol_adjoint_step: adjoint not bound");
+  hipStream_t s = (hipStream_t)stream;
+  AdjointArgs A{};
+  A.fw = make_args(c, f_state, nullptr, 0u);
+  A.lam_in = c->ab.lam[c->acur];
+  A.lam_out = c->ab.lam[1 - c->acur];
+  A.grad_perm = (c->d.options & POL_PERM_FIELD) ? c->ab.grad_perm : nullptr;
+  A.partial = c->adj_partial;
+  A.objective = objective;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_adjoint(c->d.storage, A, c->ab.rev_flags, c->grid, s);
+  if (e == hipSuccess) e = launch_adjoint_final(c->adj_partial, c->grid.nblk, c->adj_acc, s);
+  if (e != hipSuccess) return hip_fail(c, e, "pol_adjoint_step");
+  c->acur = 1 - c->acur;
+  return POL_OK;
+}
+
+int pol_adjoint_collect(pol_ctx* c, double* out4, void* stream) {
+  if (!c || !out4) return POL_ERR_INVALID;
+  if (!c->adjoint) return fail(c, POL_ERR_STATE, "pol_adjoint_collect: adjoint not bound");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = hipMemcpyAsync(out4, c->adj_acc, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_adjoint_collect");
+}
+
+}  // extern "C"

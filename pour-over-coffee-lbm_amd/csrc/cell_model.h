@@ -1,0 +1,209 @@
+// Per-cell physics of one D3Q19 step, shared by the forward stream-collide
+// kernel and the adjoint kernel (which recomputes the forward cell state).
+//
+// Populations are carried as scaled deviations D_q = 36 (f_q - w_q) in every
+// storage format and in registers.  36 w_q = c_q in {12, 2, 1} is exact, so
+//   36 (rho - 1) = sum D,   36 rho u - 18 F = sum D e   (sum c e = 0),
+//   36 (feq_q - w_q) = c_q [(rho - 1) + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u)]
+// is an exact rewrite of the raw-population algorithm in which no rounded
+// weight appears: raw FP32 f ~ w loses ~1e-7 relative per rounding (rho
+// drifts), and deviations with rounded weights w_q = 1/18, 1/36 give the
+// equilibrium momentum a systematic (1 + 7.5e-9) factor per step.
+//
+// Reference semantics followed (reference file:line):
+//   moments rho = sum f, u = (sum f e + F/2)/rho, guard rho > 1e-12
+//       src/core/legacy/lbm_solver.py:488-535
+//   equilibrium feq = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u)
+//       src/core/lbm_algorithms.py:183-218, cpu_backend.py:122-125
+//   BGK f* = f - (f - feq)/tau          src/core/backends/cpu_backend.py:129
+//   Guo forcing, textbook form of docs/mathematical/mathematical_models.md:105-121:
+//       F_q = (1 - 1/(2 tau)) w_q [3 (e_q - u).F + 9 (e_q.u)(e_q.F)]
+//       (legacy/lbm_solver.py:747-764 uses 3 e.F + 9 (e.u)(u.F) instead)
+//   tau_eff = tau + 3 nu_sgs, nu_sgs <= nu_sgs_max
+//       legacy/lbm_solver.py:565-572, les_turbulence.py:369-375
+//   Darcy-Forchheimer resistance F = -rho (nu/K) u - rho (C_F/sqrt K)|u| u
+//       filter_paper.py:471-536 (physical-unit form), applied as a Guo force
+//       with the implicit velocity of Guo & Zhao (Phys. Rev. E 66, 036304, 2002)
+//   LES disabled in the filter layer     filter_paper.py:199-204
+#pragma once
+#include "lbm_common.h"
+
+namespace pol {
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// Physics constants in the arithmetic type R.
+template <typename R>
+struct Phys {
+  R tau0;     // molecular relaxation time
+  R nu0;      // (tau0 - 1/2)/3
+  R les_c;    // 18 sqrt(2) C_s^2
+  R tau_max;  // tau0 + 3 nu_sgs_max
+  R G[3];     // constant body-force density
+  R K;        // uniform permeability
+  R CF;       // Forchheimer coefficient
+  R inlet_rho, inlet_u[3], outlet_rho;
+};
+
+// 36 (feq_q - w_q) for density rho = 1 + rd (c = 36 w_q, exact)
+template <typename R>
+__device__ inline R deq_q(int ex, int ey, int ez, R c, R rd, R rho, R ux, R uy, R uz, R usq) {
+  const R eu = R(ex) * ux + R(ey) * uy + R(ez) * uz;
+  return c * (rd + rho * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
+}
+template <typename R>
+__device__ inline R feq_q(int ex, int ey, int ez, R w, R rho, R ux, R uy, R uz, R usq) {
+  const R eu = R(ex) * ux + R(ey) * uy + R(ez) * uz;
+  return w * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq);
+}
+
+// Velocity of a porous cell with Darcy-Forchheimer resistance, solved in
+// closed form: rho u = m + (F0 + F_DF(u))/2 with
+// F_DF = -rho (nu/K) u - rho (C_F/sqrt K) |u| u gives
+// u = v / (c0 + c1 |u|), v = (m + F0/2)/rho, c0 = 1 + nu/(2K), c1 = C_F/(2 sqrt K)
+// => u = 2 v / (c0 + sqrt(c0^2 + 4 c1 |v|)).
+template <typename R>
+__device__ inline void porous_velocity(R vx, R vy, R vz, R nu, R K, R CF, R& ux, R& uy, R& uz, R& dk, R& fk) {
+  const R sqK = sqrt(K);
+  const R c0 = R(1) + nu / (R(2) * K);
+  const R c1 = CF / (R(2) * sqK);
+  const R vmag = sqrt(vx * vx + vy * vy + vz * vz);
+  const R s = R(2) / (c0 + sqrt(c0 * c0 + R(4) * c1 * vmag));
+  ux = vx * s;
+  uy = vy * s;
+  uz = vz * s;
+  dk = nu / K;    // Darcy coefficient
+  fk = CF / sqK;  // Forchheimer coefficient
+}
+
+// Moments of the collided state (for reductions / macro output).
+template <typename R>
+struct CellOut {
+  R rd;  // rho - 1
+  R ux, uy, uz;
+};
+
+// Collide one gathered cell in place: d[] in = pre-collision scaled
+// deviations D = 36 (f - w), out = post-collision ones.  kind/porous from the
+// cell flags; K = permeability, (fx,fy,fz) = extra force density.
+template <typename R, bool FULL>
+__device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous, bool les, R K, R fx, R fy, R fz,
+                                          const Phys<R>& P) {
+  CellOut<R> o;
+  if (FULL && kind == POL_KIND_INLET) {
+    const R rho = P.inlet_rho, rd = P.inlet_rho - R(1);
+    o.rd = rd;
+    o.ux = P.inlet_u[0];
+    o.uy = P.inlet_u[1];
+    o.uz = P.inlet_u[2];
+    const R usq = o.ux * o.ux + o.uy * o.uy + o.uz * o.uz;
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      d[q] = deq_q<R>(EX(q), EY(q), EZ(q), CW<R>(q), rd, rho, o.ux, o.uy, o.uz, usq);
+    });
+    return o;
+  }
+  R s36 = R(0), mx = R(0), my = R(0), mz = R(0);  // 36 (rho - 1), 36 m
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    s36 += d[q];
+    if constexpr (EX(q) > 0) mx += d[q];
+    if constexpr (EX(q) < 0) mx -= d[q];
+    if constexpr (EY(q) > 0) my += d[q];
+    if constexpr (EY(q) < 0) my -= d[q];
+    if constexpr (EZ(q) > 0) mz += d[q];
+    if constexpr (EZ(q) < 0) mz -= d[q];
+  });
+  const R rd = s36 / R(36);
+  const R rho = R(1) + rd;
+  const R rho36 = R(36) + s36;
+  const R inv36 = rho36 > R(36e-12) ? R(1) / rho36 : R(0);  // 1 / (36 rho)
+  const R inv_rho = R(36) * inv36;
+  R Fx = R(0), Fy = R(0), Fz = R(0);
+  if constexpr (FULL) {
+    Fx = P.G[0] + fx;
+    Fy = P.G[1] + fy;
+    Fz = P.G[2] + fz;
+  }
+  R ux, uy, uz;
+  if (FULL && porous) {
+    R dk, fk;
+    porous_velocity<R>((mx + R(18) * Fx) * inv36, (my + R(18) * Fy) * inv36, (mz + R(18) * Fz) * inv36, P.nu0, K,
+                       P.CF, ux, uy, uz, dk, fk);
+    const R umag = sqrt(ux * ux + uy * uy + uz * uz);
+    const R c = rho * (dk + fk * umag);
+    Fx -= c * ux;
+    Fy -= c * uy;
+    Fz -= c * uz;
+  } else {
+    ux = (mx + R(18) * Fx) * inv36;
+    uy = (my + R(18) * Fy) * inv36;
+    uz = (mz + R(18) * Fz) * inv36;
+  }
+  o.rd = rd;
+  o.ux = ux;
+  o.uy = uy;
+  o.uz = uz;
+  const R usq = ux * ux + uy * uy + uz * uz;
+  if (FULL && kind == POL_KIND_OUTLET) {
+    const R ro = P.outlet_rho, rdo = P.outlet_rho - R(1);
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      d[q] = deq_q<R>(EX(q), EY(q), EZ(q), CW<R>(q), rdo, ro, ux, uy, uz, usq);
+    });
+    return o;
+  }
+  R de[Q];
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    de[q] = deq_q<R>(EX(q), EY(q), EZ(q), CW<R>(q), rd, rho, ux, uy, uz, usq);
+  });
+  R tau = P.tau0;
+  if (FULL && les && !porous) {
+    // 36 x non-equilibrium stress: Pi_ab = sum_q e_a e_b (f_q - feq_q)
+    R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
+    static_for<1, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const R n = d[q] - de[q];
+      if constexpr (EX(q) != 0) pxx += n;
+      if constexpr (EY(q) != 0) pyy += n;
+      if constexpr (EZ(q) != 0) pzz += n;
+      if constexpr (EX(q) * EY(q) > 0) pxy += n;
+      if constexpr (EX(q) * EY(q) < 0) pxy -= n;
+      if constexpr (EX(q) * EZ(q) > 0) pxz += n;
+      if constexpr (EX(q) * EZ(q) < 0) pxz -= n;
+      if constexpr (EY(q) * EZ(q) > 0) pyz += n;
+      if constexpr (EY(q) * EZ(q) < 0) pyz -= n;
+    });
+    const R q36 = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+    // tau = (tau0 + sqrt(tau0^2 + 18 sqrt2 Cs^2 Q / rho)) / 2 with Q / rho = q36 / (36 rho)
+    tau = R(0.5) * (P.tau0 + sqrt(P.tau0 * P.tau0 + P.les_c * q36 * inv36));
+    tau = tau < P.tau_max ? tau : P.tau_max;
+  }
+  const R omega = R(1) / tau;
+  if (FULL) {
+    const R cf = R(1) - R(0.5) * omega;
+    const R uF = ux * Fx + uy * Fy + uz * Fz;
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const R eu = R(EX(q)) * ux + R(EY(q)) * uy + R(EZ(q)) * uz;
+      const R eF = R(EX(q)) * Fx + R(EY(q)) * Fy + R(EZ(q)) * Fz;
+      const R src = cf * CW<R>(q) * (R(3) * (eF - uF) + R(9) * eu * eF);
+      d[q] = d[q] - omega * (d[q] - de[q]) + src;
+    });
+  } else {
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      d[q] = d[q] - omega * (d[q] - de[q]);
+    });
+  }
+  return o;
+}
+
+}  // namespace pol

@@ -1,0 +1,71 @@
+// Internal host-side interface between the C ABI (abi.hip) and the kernel
+// translation units.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lbm_common.h"
+
+namespace pol {
+
+// Everything one stream-collide launch needs; physics kept in FP64 and
+// narrowed to the arithmetic type inside the kernel.
+struct StepArgs {
+  const void* fin;
+  void* fout;
+  const uint32_t* flags;
+  const void* force;  // real [3][N]
+  const void* perm;   // real [N]
+  void* rho;          // real [N]
+  void* u;            // real [3][N]
+  double* partial;    // [nblk][POL_NRED]
+  uint32_t N, nx, ny, nz, nxy;
+  FastDiv dnx, dny;
+  uint32_t per_xcd, nblk;
+  uint32_t options;
+  uint32_t what;
+  double tau0, nu0, les_c, tau_max;
+  double G[3];
+  double K, CF;
+  double inlet_rho, inlet_u[3], outlet_rho;
+};
+
+// Launch one pass.  store_f: write post-collision populations (a step);
+// false = moments only (macro/reduce of the current state).
+hipError_t launch_stream_collide(int storage, const StepArgs& a, bool store_f, const Grid& g, hipStream_t s);
+// Fold per-block partials into out[POL_NRED] (device), deterministic order.
+hipError_t launch_reduce_final(const double* partial, uint32_t nblk, double* out, hipStream_t s);
+// f = feq(rho, u) (rho/u real-typed, may be null)
+hipError_t launch_init_equilibrium(int storage, void* f, const void* rho, const void* u, uint32_t N, hipStream_t s);
+
+// Particle state in the real type (float, or double with real_is_double).
+struct ParticleArgs {
+  int32_t count;
+  int32_t real_is_double;
+  uint32_t nx, ny, nz, nxy, N;
+  const uint32_t* flags;
+  void *pos, *vel, *radius, *mass;
+  int32_t* active;
+  void *drag_new, *drag_old, *drag, *u_fluid, *re_p, *cd;
+  void* force;
+  const void* u_field;
+  double rho_fluid, nu_fluid, drag_max_per_mass, alpha;
+  double g[3];
+  double dt, max_disp, wall_damping;
+};
+hipError_t launch_particles_couple(const ParticleArgs& a, hipStream_t s);
+hipError_t launch_particles_advance(const ParticleArgs& a, hipStream_t s);
+hipError_t launch_particle_index(const float* pos, int32_t n, int32_t nx, int32_t ny, int32_t nz, int32_t* idx,
+                                 float* frac, hipStream_t s);
+
+struct AdjointArgs {
+  StepArgs fw;          // forward description; fw.fin = forward state f^n
+  const void* lam_in;   // adjoint of f^{n+1} (post-collision), [19][N]
+  void* lam_out;        // adjoint of f^n
+  double* grad_perm;    // [N]
+  double* partial;      // [nblk][4]: dJ/du_in x,y,z and J
+  int objective;
+};
+hipError_t launch_adjoint(int storage, const AdjointArgs& a, const uint32_t* rflags, const Grid& g, hipStream_t s);
+hipError_t launch_adjoint_final(const double* partial, uint32_t nblk, double* acc, hipStream_t s);
+
+}  // namespace pol

@@ -1,0 +1,191 @@
+// Device-side building blocks shared by the stream-collide, particle and
+// adjoint kernels: D3Q19 tables, population storage codecs, index math and
+// wave64 reductions.  gfx950 only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pourover_hip.h"
+
+namespace pol {
+
+// ---------------------------------------------------------------------------
+// D3Q19 lattice in the order of config/core.py:36-47 (reference).  Opposite
+// directions follow from e_opp = -e (the search of legacy/lbm_solver.py:431-439).
+// ---------------------------------------------------------------------------
+constexpr int Q = 19;
+__host__ __device__ constexpr int EX(int q) {
+  constexpr int t[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  return t[q];
+}
+__host__ __device__ constexpr int EY(int q) {
+  constexpr int t[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+  return t[q];
+}
+__host__ __device__ constexpr int EZ(int q) {
+  constexpr int t[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+  return t[q];
+}
+__host__ __device__ constexpr int OPP(int q) {
+  constexpr int t[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+  return t[q];
+}
+// weights 1/3, 1/18, 1/36 and their exact integer multiples c_q = 36 w_q
+template <typename R>
+__host__ __device__ constexpr R W(int q) {
+  return q == 0 ? R(1) / R(3) : (q < 7 ? R(1) / R(18) : R(1) / R(36));
+}
+template <typename R>
+__host__ __device__ constexpr R CW(int q) {
+  return q == 0 ? R(12) : (q < 7 ? R(2) : R(1));
+}
+
+// ---------------------------------------------------------------------------
+// Population storage.  Every format stores the scaled deviation
+// D_q = 36 (f_q - w_q);
+// F16/BF16 are converted with round-to-nearest-even and widened to FP32 for
+// all arithmetic.  load/store move deviations (no weight added here).
+// ---------------------------------------------------------------------------
+struct bf16_t {
+  uint16_t bits;
+};
+
+template <typename S>
+struct Codec;
+
+template <>
+struct Codec<float> {
+  using real = float;
+  __device__ static inline float load(const float* p) { return *p; }
+  __device__ static inline void store(float* p, float v) { *p = v; }
+};
+
+template <>
+struct Codec<double> {
+  using real = double;
+  __device__ static inline double load(const double* p) { return *p; }
+  __device__ static inline void store(double* p, double v) { *p = v; }
+};
+
+template <>
+struct Codec<_Float16> {
+  using real = float;
+  __device__ static inline float load(const _Float16* p) { return (float)(*p); }
+  __device__ static inline void store(_Float16* p, float v) { *p = (_Float16)v; }
+};
+
+__device__ inline float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+// round-to-nearest-even; NaN kept quiet
+__device__ inline uint16_t f32_to_bf16(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+template <>
+struct Codec<bf16_t> {
+  using real = float;
+  __device__ static inline float load(const bf16_t* p) { return bf16_to_f32(p->bits); }
+  __device__ static inline void store(bf16_t* p, float v) { p->bits = f32_to_bf16(v); }
+};
+
+// ---------------------------------------------------------------------------
+// Exact unsigned division by a runtime constant (round-up multiplier,
+// Granlund & Montgomery 1994, Thm 4.2): q = (n * m) >> (32 + l) with
+// m = ceil(2^(32+l)/d), l = ceil(log2 d); exact for n < 2^32.  For d < 2^31
+// and n < 2^31 the product n*m < 2^64 fits one 64-bit multiply.
+// ---------------------------------------------------------------------------
+struct FastDiv {
+  uint64_t m;
+  uint32_t shift;
+  uint32_t d;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  f.shift = 32 + l;
+  unsigned __int128 num = ((unsigned __int128)1) << (32 + l);
+  f.m = (uint64_t)((num + d - 1) / d);
+  return f;
+}
+__device__ __host__ inline uint32_t fdiv(uint32_t n, const FastDiv& f) {
+  return (uint32_t)(((uint64_t)n * f.m) >> f.shift);
+}
+
+// ---------------------------------------------------------------------------
+// Launch geometry: 1-D grid of 256-thread blocks over the x-fastest cell
+// index.  Hardware deals workgroups round-robin to the 8 XCDs; the remap
+// gives each XCD one contiguous run of blocks so neighbouring x/y rows (and
+// the cache lines shared by misaligned +-x pulls) stay in one L2.
+// ---------------------------------------------------------------------------
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;
+
+struct Grid {
+  uint32_t nblk;     // logical blocks covering the cells
+  uint32_t per_xcd;  // ceil(nblk / 8)
+  uint32_t launch;   // 8 * per_xcd physical blocks
+};
+inline Grid make_grid(uint64_t cells) {
+  Grid g;
+  g.nblk = (uint32_t)((cells + BLOCK - 1) / BLOCK);
+  g.per_xcd = (g.nblk + NXCD - 1) / NXCD;
+  g.launch = g.per_xcd * NXCD;
+  return g;
+}
+__device__ inline uint32_t logical_block(uint32_t per_xcd) {
+  const uint32_t b = blockIdx.x;
+  return (b % NXCD) * per_xcd + b / NXCD;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 reductions (DPP/permute via shfl_xor) in FP64
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Per-block partials of the POL_NRED reduction slots.  Slots UMAX/RHOMAX
+// reduce with max, RHOMIN with min, everything else with sum.
+__device__ inline bool red_is_max(int s) { return s == POL_RED_UMAX || s == POL_RED_RHOMAX; }
+__device__ inline bool red_is_min(int s) { return s == POL_RED_RHOMIN; }
+
+template <int NW>
+__device__ inline void block_reduce_store(double (&v)[POL_NRED], double* __restrict__ partial, uint32_t lblk) {
+  __shared__ double sh[NW][POL_NRED];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < POL_NRED; ++s) {
+    double r = red_is_max(s) ? wave_max(v[s]) : (red_is_min(s) ? wave_min(v[s]) : wave_sum(v[s]));
+    if (lane == 0) sh[wid][s] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < POL_NRED) {
+    const int s = threadIdx.x;
+    double r = sh[0][s];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) r = red_is_max(s) ? fmax(r, sh[w][s]) : (red_is_min(s) ? fmin(r, sh[w][s]) : r + sh[w][s]);
+    partial[(size_t)lblk * POL_NRED + s] = r;
+  }
+}
+
+__device__ inline double red_identity(int s) {
+  return red_is_max(s) ? -1.0e300 : (red_is_min(s) ? 1.0e300 : 0.0);
+}
+
+}  // namespace pol

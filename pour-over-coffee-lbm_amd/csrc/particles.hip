@@ -1,0 +1,264 @@
+// Lagrangian coffee-particle coupling kernels (one thread per particle),
+// templated on the real type (float for F32/F16/BF16 fluid storage, double
+// for the FP64 oracle run).
+//
+// couple : particle -> cell index + clamped trilinear fractions, gather of the
+//          fluid velocity at the 8 corners, Schiller-Naumann drag, drag
+//          under-relaxation, and the reaction -drag spread to the 8 corners
+//          with hardware floating-point atomics into the force-density field
+//          that the next stream-collide step reads.
+// advance: zero this step's 8-corner footprint in the force field (so the
+//          field never needs a full-grid clear), then move the particle
+//          (linearised-implicit drag, buoyancy-corrected gravity, displacement
+//          clamp, rejection at walls and at the filter layer, removal at
+//          the outlet).
+//
+// Reference (coffee_particles.py): index/weights :1159-1174 (dupes
+// :1008-1022, :1052-1066), gather :1176-1196, drag coefficient :1088-1099,
+// drag + scatter :1107-1154 and :1048-1086, under-relaxation :1200-1212,
+// clear :1101-1105, particle update :641-720.
+#include "engine.h"
+
+namespace pol {
+
+// i = (int)max(0, min(n - 2, p)) with min/max in the position's type (the
+// reference's ti.min/ti.max promote the int bound to f32) and truncation,
+// frac = clamp(p - i, 0, 1).  fmin/fmax return the non-NaN operand, so a NaN
+// coordinate maps to n-2.
+__device__ __forceinline__ void cell_index(float p, int n, int& i, float& fr) {
+  const float c = fmaxf(0.0f, fminf((float)(n - 2), p));
+  i = (int)c;
+  const float t = p - (float)i;
+  fr = fmaxf(0.0f, fminf(1.0f, t));
+}
+__device__ __forceinline__ void cell_index(double p, int n, int& i, double& fr) {
+  const double c = fmax(0.0, fmin((double)(n - 2), p));
+  i = (int)c;
+  const double t = p - (double)i;
+  fr = fmax(0.0, fmin(1.0, t));
+}
+
+// Schiller-Naumann with the reference's piecewise bounds:
+//   Re < 0.1        : 24 / max(0.01, Re)
+//   0.1 <= Re < 1000: 24/Re (1 + 0.15 Re^0.687)
+//   Re >= 1000      : 0.44
+template <typename R>
+__device__ __forceinline__ R drag_coefficient(R re) {
+  R cd = R(24) / fmax(R(0.01), re);
+  if (re >= R(0.1) && re < R(1000)) cd = R(24) / re * (R(1) + R(0.15) * pow(re, R(0.687)));
+  else if (re >= R(1000)) cd = R(0.44);
+  return cd;
+}
+
+__device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.count) return;
+  const int32_t* active = a.active;
+  if (!active[p]) return;
+  const R* pos = (const R*)a.pos;
+  const R* vel = (const R*)a.vel;
+  const R* ufield = (const R*)a.u_field;
+  R* force = (R*)a.force;
+  const int P = a.count;
+  const R px = pos[p], py = pos[P + p], pz = pos[2 * P + p];
+  int i, j, k;
+  R fx, fy, fz;
+  cell_index(px, (int)a.nx, i, fx);
+  cell_index(py, (int)a.ny, j, fy);
+  cell_index(pz, (int)a.nz, k, fz);
+  const R w000 = (1 - fx) * (1 - fy) * (1 - fz);
+  const R w001 = (1 - fx) * (1 - fy) * fz;
+  const R w010 = (1 - fx) * fy * (1 - fz);
+  const R w011 = (1 - fx) * fy * fz;
+  const R w100 = fx * (1 - fy) * (1 - fz);
+  const R w101 = fx * (1 - fy) * fz;
+  const R w110 = fx * fy * (1 - fz);
+  const R w111 = fx * fy * fz;
+  const uint32_t nx = a.nx, nxy = a.nxy, N = a.N;
+  const uint32_t c000 = (uint32_t)k * nxy + (uint32_t)j * nx + (uint32_t)i;
+  const uint32_t c001 = c000 + nxy, c010 = c000 + nx, c011 = c000 + nx + nxy;
+  const uint32_t c100 = c000 + 1, c101 = c001 + 1, c110 = c010 + 1, c111 = c011 + 1;
+  R uf[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const R* u = ufield + (size_t)d * N;
+    uf[d] = w000 * u[c000] + w001 * u[c001] + w010 * u[c010] + w011 * u[c011] + w100 * u[c100] + w101 * u[c101] +
+            w110 * u[c110] + w111 * u[c111];
+    ((R*)a.u_fluid)[(size_t)d * P + p] = uf[d];
+  }
+  const R rx = uf[0] - vel[p], ry = uf[1] - vel[P + p], rz = uf[2] - vel[2 * P + p];
+  const R mag = sqrt(rx * rx + ry * ry + rz * rz);
+  R dn[3] = {R(0), R(0), R(0)};
+  R* re_p = (R*)a.re_p;
+  R* cdp = (R*)a.cd;
+  if (mag > R(1e-8)) {
+    const R r = ((const R*)a.radius)[p];
+    const R rho_f = (R)a.rho_fluid;
+    const R mu = rho_f * (R)a.nu_fluid;
+    const R re = rho_f * mag * R(2) * r / fmax(R(1e-8), mu);
+    const R cd = drag_coefficient<R>(re);
+    re_p[p] = re;
+    cdp[p] = cd;
+    const R area = R(3.14159) * r * r;  // the reference's literal
+    R dmag = R(0.5) * rho_f * cd * area * mag;
+    dmag = fmin(dmag, ((const R*)a.mass)[p] * (R)a.drag_max_per_mass);
+    const R s = dmag / mag;
+    dn[0] = s * rx;
+    dn[1] = s * ry;
+    dn[2] = s * rz;
+    // Newton's third law: -drag onto the 8 corners
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R* F = force + (size_t)d * N;
+      const R rf = -dn[d];
+      atomic_add(F + c000, w000 * rf);
+      atomic_add(F + c010, w010 * rf);
+      atomic_add(F + c001, w001 * rf);
+      atomic_add(F + c011, w011 * rf);
+      atomic_add(F + c100, w100 * rf);
+      atomic_add(F + c110, w110 * rf);
+      atomic_add(F + c101, w101 * rf);
+      atomic_add(F + c111, w111 * rf);
+    }
+  } else {
+    re_p[p] = R(0);
+    cdp[p] = R(0);
+  }
+  const R al = (R)a.alpha;
+  R* drag_new = (R*)a.drag_new;
+  R* drag_old = (R*)a.drag_old;
+  R* drag = (R*)a.drag;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    drag_new[(size_t)d * P + p] = dn[d];
+    const R rel = al * dn[d] + (R(1) - al) * drag_old[(size_t)d * P + p];
+    drag[(size_t)d * P + p] = rel;
+    drag_old[(size_t)d * P + p] = rel;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_particles_advance(const ParticleArgs a) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.count) return;
+  int32_t* active = a.active;
+  if (!active[p]) return;
+  R* pos = (R*)a.pos;
+  R* vel = (R*)a.vel;
+  R* force = (R*)a.force;
+  const int P = a.count;
+  const uint32_t nx = a.nx, ny = a.ny, nz = a.nz, nxy = a.nxy, N = a.N;
+  const R x[3] = {pos[p], pos[P + p], pos[2 * P + p]};
+  // 1. clear the reaction footprint written by couple (same corners)
+  {
+    int i, j, k;
+    R t;
+    cell_index(x[0], (int)nx, i, t);
+    cell_index(x[1], (int)ny, j, t);
+    cell_index(x[2], (int)nz, k, t);
+    const uint32_t c000 = (uint32_t)k * nxy + (uint32_t)j * nx + (uint32_t)i;
+    const uint32_t cs[8] = {c000, c000 + 1, c000 + nx, c000 + nx + 1, c000 + nxy, c000 + nxy + 1, c000 + nxy + nx,
+                            c000 + nxy + nx + 1};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) force[(size_t)d * N + cs[c]] = R(0);
+  }
+  // 2. velocity: drag linearised about the current relative velocity,
+  //    dv/dt = beta/m (u_f - v) + g, implicit in beta (stable for any
+  //    particle response time)
+  const R* ufl = (const R*)a.u_fluid;
+  const R* dnp = (const R*)a.drag_new;
+  R v[3] = {vel[p], vel[P + p], vel[2 * P + p]};
+  const R uf[3] = {ufl[p], ufl[P + p], ufl[2 * P + p]};
+  const R dn[3] = {dnp[p], dnp[P + p], dnp[2 * P + p]};
+  const R rx = uf[0] - v[0], ry = uf[1] - v[1], rz = uf[2] - v[2];
+  const R mag = sqrt(rx * rx + ry * ry + rz * rz);
+  const R dmag = sqrt(dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2]);
+  const R m = ((const R*)a.mass)[p];
+  const R beta = mag > R(1e-8) ? dmag / mag : R(0);
+  const R dt = (R)a.dt;
+  const R h = dt * beta / m;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) v[d] = (v[d] + h * uf[d] + dt * (R)a.g[d]) / (R(1) + h);
+  // 3. position with displacement clamp
+  R dx[3] = {dt * v[0], dt * v[1], dt * v[2]};
+  const R dl = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+  const R max_disp = (R)a.max_disp;
+  if (dl > max_disp) {
+    const R s = max_disp / dl;
+    dx[0] *= s;
+    dx[1] *= s;
+    dx[2] *= s;
+  }
+  R xn[3] = {x[0] + dx[0], x[1] + dx[1], x[2] + dx[2]};
+  // 4. boundaries: leave through the outlet / bottom; reject moves into walls
+  //    or the filter layer (the paper retains the grounds)
+  if (!(xn[2] >= R(1)) || !(xn[0] >= R(0)) || !(xn[1] >= R(0)) || !(xn[0] < (R)(nx - 1)) ||
+      !(xn[1] < (R)(ny - 1)) || !(xn[2] < (R)(nz - 1))) {
+    active[p] = 0;
+  } else {
+    const uint32_t c = (uint32_t)xn[2] * nxy + (uint32_t)xn[1] * nx + (uint32_t)xn[0];
+    const uint32_t fl = a.flags ? a.flags[c] : 0u;
+    const uint32_t kd = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;
+    if (kd == POL_KIND_OUTLET) {
+      active[p] = 0;
+    } else if (kd == POL_KIND_SOLID || (fl & POL_FLAG_POROUS)) {
+      const R wd = (R)a.wall_damping;
+      for (int d = 0; d < 3; ++d) {
+        xn[d] = x[d];
+        v[d] *= wd;
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    pos[(size_t)d * P + p] = xn[d];
+    vel[(size_t)d * P + p] = v[d];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_particle_index(const float* __restrict__ pos, int32_t n, int32_t nx,
+                                                        int32_t ny, int32_t nz, int32_t* __restrict__ idx,
+                                                        float* __restrict__ frac) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const int dims[3] = {nx, ny, nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int i;
+    float t;
+    cell_index(pos[(size_t)d * n + p], dims[d], i, t);
+    idx[(size_t)d * n + p] = i;
+    frac[(size_t)d * n + p] = t;
+  }
+}
+
+hipError_t launch_particles_couple(const ParticleArgs& a, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  if (a.real_is_double)
+    hipLaunchKernelGGL(k_particles_couple<double>, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_particles_couple<float>, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_particles_advance(const ParticleArgs& a, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  if (a.real_is_double)
+    hipLaunchKernelGGL(k_particles_advance<double>, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_particles_advance<float>, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_particle_index(const float* pos, int32_t n, int32_t nx, int32_t ny, int32_t nz, int32_t* idx,
+                                 float* frac, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_particle_index, dim3((n + 255) / 256), dim3(256), 0, s, pos, n, nx, ny, nz, idx, frac);
+  return hipGetLastError();
+}
+
+}  // namespace pol

@@ -1,0 +1,218 @@
+"""Host-side driver of one HIP solver context (device buffers + C ABI calls).
+
+Torch-ROCm tensors are only the device allocations; every computation on
+them is a launch of libpourover_hip.so through the C ABI (_lib.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+_STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
+
+
+@dataclass
+class SolverConfig:
+    """Descriptor of one lattice (fields of pol_desc plus buffer options)."""
+
+    nx: int
+    ny: int
+    nz: int
+    precision: str = "f32"           # population storage: f32 | f16 | bf16 | f64
+    tau: float = 0.6
+    periodic: Sequence[bool] = (True, True, True)
+    les: bool = False
+    cs_smag: float = 0.17
+    nu_sgs_max: float = 0.1          # les_turbulence.py:375
+    gravity: Optional[Sequence[float]] = None
+    darcy_K: float = 1e-3
+    forch_CF: float = 0.55
+    inlet_rho: float = 1.0
+    inlet_u: Sequence[float] = (0.0, 0.0, 0.0)
+    outlet_rho: float = 1.0
+    force_field: bool = False
+    perm_field: bool = False
+    device: int = 0
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def real_dtype(self) -> torch.dtype:
+        return torch.float64 if self.precision == "f64" else torch.float32
+
+    @property
+    def storage_dtype(self) -> torch.dtype:
+        return _STORAGE_TORCH[self.precision]
+
+    @property
+    def bytes_per_population(self) -> int:
+        return {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}[self.precision]
+
+    def options(self, use_flags: bool) -> int:
+        o = 0
+        px, py, pz = self.periodic
+        o |= L.POL_PERIODIC_X if px else 0
+        o |= L.POL_PERIODIC_Y if py else 0
+        o |= L.POL_PERIODIC_Z if pz else 0
+        o |= L.POL_USE_FLAGS if use_flags else 0
+        o |= L.POL_LES if self.les else 0
+        o |= L.POL_FORCE_FIELD if self.force_field else 0
+        o |= L.POL_PERM_FIELD if self.perm_field else 0
+        o |= L.POL_GRAVITY if self.gravity is not None else 0
+        return o
+
+    def desc(self, use_flags: bool) -> L.PolDesc:
+        d = L.PolDesc()
+        d.nx, d.ny, d.nz = self.nx, self.ny, self.nz
+        d.storage = L.STORAGE_CODES[self.precision]
+        d.options = self.options(use_flags)
+        d.device = self.device
+        d.tau = self.tau
+        d.cs_smag = self.cs_smag
+        d.nu_sgs_max = self.nu_sgs_max
+        d.gravity = (C.c_double * 3)(*(self.gravity or (0.0, 0.0, 0.0)))
+        d.darcy_K = self.darcy_K
+        d.forch_CF = self.forch_CF
+        d.inlet_rho = self.inlet_rho
+        d.inlet_u = (C.c_double * 3)(*self.inlet_u)
+        d.outlet_rho = self.outlet_rho
+        return d
+
+    def oracle_dict(self, use_flags: bool) -> dict:
+        """Keyword form of the same descriptor (for checkers)."""
+        return dict(nx=self.nx, ny=self.ny, nz=self.nz, tau=self.tau, options=self.options(use_flags),
+                    cs_smag=self.cs_smag, nu_sgs_max=self.nu_sgs_max,
+                    gravity=tuple(self.gravity or (0.0, 0.0, 0.0)), darcy_K=self.darcy_K, forch_CF=self.forch_CF,
+                    inlet_rho=self.inlet_rho, inlet_u=tuple(self.inlet_u), outlet_rho=self.outlet_rho)
+
+
+def _as_int32_view(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32))
+
+
+class HipContext:
+    """One pol_ctx with its device buffers."""
+
+    def __init__(self, cfg: SolverConfig, flags: Optional[np.ndarray] = None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("HipContext needs a visible HIP device (torch.cuda.is_available() is False)")
+        self.lib = L.load()
+        self.config = cfg
+        self.device = torch.device("cuda", cfg.device)
+        shape = (cfg.nz, cfg.ny, cfg.nx)
+        self.shape = shape
+        self.cells = cfg.cells
+        sd, rd = cfg.storage_dtype, cfg.real_dtype
+        # zero = rest state f = w; solid cells of the write buffer are never written
+        self.f = [torch.zeros((19,) + shape, dtype=sd, device=self.device) for _ in range(2)]
+        self.rho = torch.ones(shape, dtype=rd, device=self.device)
+        self.u = torch.zeros((3,) + shape, dtype=rd, device=self.device)
+        self.flags = _as_int32_view(flags).to(self.device) if flags is not None else None
+        self.force = torch.zeros((3,) + shape, dtype=rd, device=self.device) if cfg.force_field else None
+        self.perm = torch.full(shape, cfg.darcy_K, dtype=rd, device=self.device) if cfg.perm_field else None
+        self._ctx = C.c_void_p()
+        self._use_flags = flags is not None
+        st = self.lib.pol_create(C.byref(self._ctx), C.byref(cfg.desc(self._use_flags)))
+        L.check(st, None, "pol_create")
+        self._bind()
+
+    # -- lifecycle -----------------------------------------------------------
+    def _bind(self) -> None:
+        b = L.PolBuffers()
+        b.f[0] = self.f[0].data_ptr()
+        b.f[1] = self.f[1].data_ptr()
+        b.cell_flags = self.flags.data_ptr() if self.flags is not None else None
+        b.force = self.force.data_ptr() if self.force is not None else None
+        b.perm = self.perm.data_ptr() if self.perm is not None else None
+        b.rho = self.rho.data_ptr()
+        b.u = self.u.data_ptr()
+        L.check(self.lib.pol_bind(self._ctx, C.byref(b)), self._ctx, "pol_bind")
+
+    def close(self) -> None:
+        if self._ctx:
+            self.lib.pol_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._ctx
+
+    # -- state -----------------------------------------------------------------
+    @property
+    def current(self) -> int:
+        i = C.c_int(0)
+        L.check(self.lib.pol_current(self._ctx, C.byref(i)), self._ctx, "pol_current")
+        return i.value
+
+    def set_current(self, i: int) -> None:
+        L.check(self.lib.pol_set_current(self._ctx, int(i)), self._ctx, "pol_set_current")
+
+    @property
+    def f_current(self) -> torch.Tensor:
+        return self.f[self.current]
+
+    @property
+    def f_next(self) -> torch.Tensor:
+        return self.f[1 - self.current]
+
+    def set_flags(self, flags: np.ndarray) -> None:
+        if not self._use_flags:
+            raise ValueError("context was created without cell flags")
+        self.flags.copy_(_as_int32_view(flags).to(self.device))
+
+    def synchronize(self) -> None:
+        torch.cuda.synchronize(self.device)
+
+    # -- launches --------------------------------------------------------------
+    def init_equilibrium(self, rho: Optional[torch.Tensor] = None, u: Optional[torch.Tensor] = None) -> None:
+        rd = self.config.real_dtype
+        r = rho.to(self.device, rd).contiguous() if rho is not None else None
+        uu = u.to(self.device, rd).contiguous() if u is not None else None
+        st = self.lib.pol_init_equilibrium(self._ctx, L.ptr(r), L.ptr(uu), L.stream_handle())
+        L.check(st, self._ctx, "pol_init_equilibrium")
+        if r is not None or uu is not None:
+            torch.cuda.current_stream().synchronize()  # r/uu temporaries die here
+
+    def step(self, n: int = 1, what: int = 0) -> Optional[np.ndarray]:
+        red = np.zeros(L.POL_NRED, dtype=np.float64) if what & L.POL_WANT_REDUCE else None
+        st = self.lib.pol_step_ex(self._ctx, int(n), int(what),
+                                  red.ctypes.data_as(C.c_void_p) if red is not None else None, L.stream_handle())
+        L.check(st, self._ctx, "pol_step")
+        return red
+
+    def step_from(self, fin: torch.Tensor, fout: torch.Tensor, what: int = 0) -> Optional[np.ndarray]:
+        red = np.zeros(L.POL_NRED, dtype=np.float64) if what & L.POL_WANT_REDUCE else None
+        st = self.lib.pol_step_from(self._ctx, L.ptr(fin), L.ptr(fout), int(what),
+                                    red.ctypes.data_as(C.c_void_p) if red is not None else None, L.stream_handle())
+        L.check(st, self._ctx, "pol_step_from")
+        return red
+
+    def macroscopic(self, macro: bool = True, reduce: bool = False) -> Optional[np.ndarray]:
+        what = (L.POL_WANT_MACRO if macro else 0) | (L.POL_WANT_REDUCE if reduce else 0)
+        red = np.zeros(L.POL_NRED, dtype=np.float64) if reduce else None
+        st = self.lib.pol_macroscopic(self._ctx, what, red.ctypes.data_as(C.c_void_p) if red is not None else None,
+                                      L.stream_handle())
+        L.check(st, self._ctx, "pol_macroscopic")
+        return red
+
+    def reduce(self) -> dict:
+        red = self.macroscopic(macro=False, reduce=True)
+        return reduction_dict(red)
+
+
+def reduction_dict(red: np.ndarray) -> dict:
+    return {k: float(v) for k, v in zip(L.RED_SLOTS, red)}

@@ -1,0 +1,136 @@
+"""V60 geometry and flag words (CPU): integer results pinned by invariants and
+committed digests (tests/golden/golden.json, written by tests/golden/make_golden.py)."""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from pourover_amd import _lib as L
+from pourover_amd.geometry import V60Params, build_v60, link_flags
+from pourover_amd.lattice import E, OPP, Q
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "golden.json")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(GOLDEN) as fh:
+        return json.load(fh)
+
+
+@pytest.fixture(scope="module")
+def g48():
+    return build_v60(V60Params.scaled(48))
+
+
+def _kind(flags):
+    return (flags >> L.POL_KIND_SHIFT) & L.POL_KIND_MASK
+
+
+@pytest.mark.parametrize("n", [32, 48, 128])
+def test_digest_matches_golden(golden, n):
+    g = build_v60(V60Params() if n == 224 else V60Params.scaled(n))
+    ref = golden["geometry"][str(n)]
+    assert g.counts() == ref["counts"]
+    assert g.digest() == ref["sha256"]
+
+
+def test_digest_224(golden):
+    g = build_v60()
+    ref = golden["geometry"]["224"]
+    assert g.counts() == ref["counts"]
+    assert g.digest() == ref["sha256"]
+    # BASELINE geometry: outlet hole radius 8 -> outlet plane = discrete disk r <= 8 around (111.5, 111.5)
+    yy, xx = np.mgrid[0:224, 0:224]
+    disk = ((xx - 111.5) ** 2 + (yy - 111.5) ** 2) <= 64.0
+    assert g.counts()["outlet"] == int(disk.sum())
+    jet = ((xx - 111.5) ** 2 + (yy - 111.5) ** 2) <= 16.0
+    assert g.counts()["inlet"] == int(jet.sum())
+
+
+def _check_no_solid_gather(flags, periodic):
+    kind = _kind(flags)
+    nz, ny, nx = kind.shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    gathers = (kind != L.POL_KIND_SOLID) & (kind != L.POL_KIND_INLET)
+    dims = (nx, ny, nz)
+    for q in range(1, Q):
+        bounce = (flags >> np.uint32(q)) & 1
+        src = []
+        for c, e, n, per in ((x, E[q, 0], nx, periodic[0]), (y, E[q, 1], ny, periodic[1]), (z, E[q, 2], nz, periodic[2])):
+            s = c - e
+            s = np.mod(s, n) if per else np.clip(s, 0, n - 1)
+            src.append(s)
+        src_kind = kind[src[2], src[1], src[0]]
+        bad = gathers & (bounce == 0) & (src_kind == L.POL_KIND_SOLID)
+        assert not bad.any(), "link %d gathers a solid cell at %s" % (q, np.argwhere(bad)[:3])
+
+
+def test_no_gather_reads_a_solid_cell(g48):
+    _check_no_solid_gather(g48.flags, (False, False, False))
+
+
+def test_bounce_bits_only_on_open_cells_and_mirror_walls(g48):
+    kind = _kind(g48.flags)
+    solid = kind == L.POL_KIND_SOLID
+    assert ((g48.flags[solid] & L.POL_LINK_MASK) == 0).all()
+    assert ((g48.flags & 1) == 0).all()          # rest population never bounces
+
+
+def test_periodic_box_has_no_walls():
+    flags, rev = link_flags(np.zeros((6, 5, 4), dtype=np.uint8), periodic=(True, True, True))
+    assert (flags == 0).all()
+    assert (rev == (1 << Q) - 1).all()           # every population has its normal reader
+
+
+def test_outlet_and_inlet_planes(g48):
+    kind = _kind(g48.flags)
+    assert (kind[0][kind[0] != L.POL_KIND_SOLID] == L.POL_KIND_OUTLET).all()
+    assert (kind[-1][kind[-1] != L.POL_KIND_SOLID] == L.POL_KIND_INLET).all()
+    assert (kind[1:-1] != L.POL_KIND_OUTLET).all() and (kind[1:-1] != L.POL_KIND_INLET).all()
+
+
+@pytest.mark.parametrize("n", [48, 224])
+def test_filter_separates_inlet_from_outlet(n):
+    """Every D3Q19 path from the jet to the outlet crosses the porous paper:
+    the non-porous fluid splits (18-connectivity = the D3Q19 links) into a
+    component holding the inlet and one holding the outlet."""
+    g = build_v60(V60Params() if n == 224 else V60Params.scaled(n))
+    kind = g.kind
+    open_fluid = (kind != L.POL_KIND_SOLID) & ~g.porous
+    lab, _ = ndimage.label(open_fluid, structure=ndimage.generate_binary_structure(3, 2))
+    lin = set(np.unique(lab[kind == L.POL_KIND_INLET])) - {0}
+    lout = set(np.unique(lab[kind == L.POL_KIND_OUTLET])) - {0}
+    assert lin and lout and not (lin & lout)
+    # ... while the whole non-solid region (paper included) is connected
+    lab2, _ = ndimage.label(kind != L.POL_KIND_SOLID, structure=ndimage.generate_binary_structure(3, 2))
+    a = set(np.unique(lab2[kind == L.POL_KIND_INLET]))
+    b = set(np.unique(lab2[kind == L.POL_KIND_OUTLET]))
+    assert a == b
+
+
+def test_reverse_flags_are_the_transpose_of_gathers(g48):
+    """rev bit p of y set  <=>  cell y + e_p gathers population p of y by a normal link."""
+    flags, rev = g48.flags, g48.rev_flags
+    kind = _kind(flags)
+    nz, ny, nx = kind.shape
+    for p in range(Q):
+        ex, ey, ez = E[p]
+        zs, ys, xs = np.nonzero((rev >> np.uint32(p)) & 1)
+        rz, ry, rx = zs + ez, ys + ey, xs + ex
+        assert ((rz >= 0) & (rz < nz) & (ry >= 0) & (ry < ny) & (rx >= 0) & (rx < nx)).all()
+        rk = kind[rz, ry, rx]
+        assert ((rk != L.POL_KIND_SOLID) & (rk != L.POL_KIND_INLET)).all()
+        assert (((flags[rz, ry, rx] >> np.uint32(p)) & 1) == 0).all()
+        assert (kind[zs, ys, xs] != L.POL_KIND_SOLID).all()
+
+
+def test_porous_cells_have_les_off(g48):
+    por = (g48.flags & L.POL_FLAG_POROUS) != 0
+    assert por.any()
+    assert ((g48.flags[por] & L.POL_FLAG_NO_LES) != 0).all()
+    assert (_kind(g48.flags)[por] == L.POL_KIND_FLUID).all()

@@ -1,0 +1,137 @@
+"""GPU parity of the fused stream-collide kernel against the C oracle.
+
+Every call goes through the C ABI (engine.HipContext -> libpourover_hip.so).
+Bars: FP32 storage/arithmetic within 1e-5 relative L2 on rho and u of the
+FP64 oracle, FP64 within 1e-12.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+from pourover_amd import _lib as L
+from pourover_amd import workloads as WL
+from pourover_amd.engine import HipContext
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_run(cfg, steps, flags=None, rho0=None, u0=None):
+    ctx = HipContext(cfg, flags)
+    ctx.init_equilibrium(None if rho0 is None else torch.from_numpy(rho0),
+                         None if u0 is None else torch.from_numpy(u0))
+    ctx.step(steps)
+    red = ctx.macroscopic(macro=True, reduce=True)
+    return ctx, ctx.rho.double().cpu().numpy(), ctx.u.double().cpu().numpy(), red
+
+
+def _oracle_run(O, cfg, steps, flags=None, rho0=None, u0=None, perm=None, force=None):
+    d = O.desc_from_dict(cfg.oracle_dict(flags is not None))
+    f = O.init_equilibrium(cfg.nx, cfg.ny, cfg.nz, rho0, u0)
+    for _ in range(steps):
+        f, _, _, _ = O.step(d, f, flags, force=force, perm=perm)
+    rho, u, red = O.moments(d, f, flags, force=force, perm=perm)
+    return f, rho, u, red
+
+
+def test_taylor_green_64_f32_1000_steps(hip, oracle):
+    """Config 1: 64^3 periodic TG, tau 0.6, u0 0.05, 1000 steps."""
+    n, steps = 64, 1000
+    cfg = WL.taylor_green_config(n, "f32")
+    rho0, u0 = WL.taylor_green_fields(n)
+    print("case: taylor_green 64^3 f32 1000 steps", flush=True)
+    _, rho_g, u_g, red = _gpu_run(cfg, steps, rho0=rho0, u0=u0)
+    _, rho_o, u_o, red_o = _oracle_run(oracle, cfg, steps, rho0=rho0, u0=u0)
+    eu, er = rel_l2(u_g, u_o), rel_l2(rho_g, rho_o)
+    print("rel L2 u %.3e rho %.3e" % (eu, er))
+    assert eu < 1e-5
+    assert er < 1e-5
+    # reductions of the kernel epilogue vs the oracle's FP64 sums
+    assert abs(red[0] - red_o[0]) / red_o[0] < 1e-9           # mass
+    assert abs(red[4] - red_o[4]) / red_o[4] < 1e-5           # kinetic energy
+    # mass conserved exactly up to round-off (periodic, no source)
+    assert abs(red[0] - n**3) / n**3 < 1e-9
+
+
+def test_taylor_green_linear_decay_analytic(hip):
+    """Low-amplitude TG (u0 = 0.005, Re ~ 1.5): KE(t) = KE0 exp(-2 nu |k|^2 t) to the
+    O(k^2) lattice error (1.4 % at 64^3, t = 1000).  At u0 = 0.05 (Re ~ 15) the
+    nonlinear transfer lowers KE by ~23 % at t = 1000, so the closed form is
+    pinned here, not on the config-1 amplitude."""
+    n, steps, u0 = 64, 1000, 0.005
+    cfg = WL.taylor_green_config(n, "f32")
+    rho0, uu = WL.taylor_green_fields(n, u0)
+    _, _, _, red = _gpu_run(cfg, steps, rho0=rho0, u0=uu)
+    ke_an = WL.taylor_green_ke(n, u0, 0.6, steps)
+    assert abs(red[4] / ke_an - 1.0) < 0.02
+
+
+def test_taylor_green_f64_bitwise_close(hip, oracle):
+    n, steps = 32, 100
+    cfg = WL.taylor_green_config(n, "f64")
+    rho0, u0 = WL.taylor_green_fields(n)
+    _, rho_g, u_g, _ = _gpu_run(cfg, steps, rho0=rho0, u0=u0)
+    _, rho_o, u_o, _ = _oracle_run(oracle, cfg, steps, rho0=rho0, u0=u0)
+    assert rel_l2(u_g, u_o) < 1e-12
+    assert rel_l2(rho_g, rho_o) < 1e-13
+
+
+@pytest.mark.parametrize("n,steps", [(40, 300)])
+def test_v60_scaled_full_physics_f32(hip, oracle, n, steps):
+    """Config-2 physics (walls, porous filter, LES, gravity, inlet, outlet) on a scaled cup."""
+    wl = WL.v60_config(n, "f32")
+    flags = wl.geometry.flags
+    print("case: v60 %d^3 f32 %d steps" % (n, steps), flush=True)
+    _, rho_g, u_g, red = _gpu_run(wl.config, steps, flags)
+    _, rho_o, u_o, red_o = _oracle_run(oracle, wl.config, steps, flags)
+    eu, er = rel_l2(u_g, u_o), rel_l2(rho_g, rho_o)
+    print("rel L2 u %.3e rho %.3e outflux gpu %.6e oracle %.6e" % (eu, er, red[7], red_o[7]))
+    assert eu < 1e-5
+    assert er < 1e-5
+    assert red[L.RED_SLOTS.index("nonfinite_cells")] == 0
+    assert abs(red[7] - red_o[7]) <= 1e-4 * abs(red_o[7]) + 1e-12
+
+
+def test_v60_scaled_f64_matches_oracle(hip, oracle):
+    wl = WL.v60_config(32, "f64")
+    flags = wl.geometry.flags
+    _, rho_g, u_g, _ = _gpu_run(wl.config, 100, flags)
+    _, rho_o, u_o, _ = _oracle_run(oracle, wl.config, 100, flags)
+    assert rel_l2(u_g, u_o) < 1e-11
+    assert rel_l2(rho_g, rho_o) < 1e-13
+
+
+def test_permeability_field_and_force_field(hip, oracle):
+    """Per-cell K (adjoint workload input) and an external force-density field."""
+    wl = WL.v60_config(32, "f32", perm_field=True, force_field=True)
+    flags = wl.geometry.flags
+    K = WL.permeability_field(wl.geometry)
+    rng = np.random.default_rng(7)
+    force = (rng.standard_normal((3,) + flags.shape) * 1e-6)
+    ctx = HipContext(wl.config, flags)
+    ctx.perm.copy_(torch.from_numpy(K))
+    ctx.force.copy_(torch.from_numpy(force))
+    ctx.init_equilibrium()
+    ctx.step(60)
+    ctx.macroscopic(macro=True)
+    _, rho_o, u_o, _ = _oracle_run(oracle, wl.config, 60, flags, perm=K, force=force)
+    assert rel_l2(ctx.u.double().cpu().numpy(), u_o) < 1e-5
+    assert rel_l2(ctx.rho.double().cpu().numpy(), rho_o) < 1e-5
+
+
+def test_step_from_and_current_buffer(hip):
+    """pol_step_from writes an explicit buffer and leaves the ping-pong index alone."""
+    cfg = WL.taylor_green_config(16, "f32")
+    rho0, u0 = WL.taylor_green_fields(16)
+    ctx = HipContext(cfg)
+    ctx.init_equilibrium(torch.from_numpy(rho0), torch.from_numpy(u0))
+    a = ctx.f_current.clone()
+    out = torch.empty_like(a)
+    ctx.step_from(a, out)
+    assert ctx.current == 0
+    ctx.step(1)
+    assert ctx.current == 1
+    torch.cuda.synchronize()
+    assert torch.equal(out, ctx.f_current)
