@@ -177,9 +177,17 @@ int pol_macroscopic(pol_ctx *ctx, uint32_t what, double *red_out, void *stream);
 int pol_reduce(pol_ctx *ctx, double *red_out, void *stream);
 
 /* ---- Lagrangian coffee particles (src/physics/coffee_particles.py) ------- */
+/* drag magnitude laws (C_D from the Schiller-Naumann piecewise form of
+ * coffee_particles.py:1088-1099 in both):
+ *   TEXTBOOK : |F| = 1/2 rho C_D pi r^2 |u_rel|^2  (docs/mathematical/mathematical_models.md:250-275;
+ *              Stokes 3 pi mu d |u_rel| at low Re)
+ *   REFERENCE: |F| = 1/2 rho C_D pi r^2 |u_rel|    (coffee_particles.py:1137-1145 as written) */
+#define POL_DRAG_TEXTBOOK 0
+#define POL_DRAG_REFERENCE 1
+
 typedef struct pol_particle_desc {
   int32_t count;         /* number of particle slots */
-  int32_t reserved;
+  int32_t drag_law;      /* POL_DRAG_TEXTBOOK or POL_DRAG_REFERENCE */
   double rho_fluid;      /* fluid density in drag law */
   double nu_fluid;       /* kinematic viscosity (mu = rho_fluid * nu_fluid) */
   double drag_max_per_mass; /* |drag| <= this * mass */

@@ -362,6 +362,7 @@ real SFX(oracle_drag_coefficient)(real re) { return drag_cd(re); }
 
 typedef struct {
   int count;
+  int drag_law; /* POL_DRAG_TEXTBOOK: |F| ~ |u_rel|^2; POL_DRAG_REFERENCE: ~ |u_rel| as coded in the reference */
   real rho_fluid, nu_fluid, drag_max_per_mass, alpha, g[3], dt, max_disp, wall_damping;
 } oracle_particle_params;
 
@@ -415,6 +416,7 @@ void SFX(oracle_particles_couple)(int nx, int ny, int nz, const oracle_particle_
       cd_out[p] = cd;
       real area = (real)3.14159 * r * r;
       real dm = (real)0.5 * pp->rho_fluid * cd * area * mag;
+      if (pp->drag_law == POL_DRAG_TEXTBOOK) dm *= mag;
       real cap = mass[p] * pp->drag_max_per_mass;
       if (dm > cap) dm = cap;
       for (int d = 0; d < 3; ++d) dn[d] = dm * rel[d] / mag;

@@ -38,7 +38,8 @@ class ParticleParams(C.Structure):
 
 
 def _pp_fields(real):
-    return [("count", C.c_int), ("rho_fluid", real), ("nu_fluid", real), ("drag_max_per_mass", real),
+    return [("count", C.c_int), ("drag_law", C.c_int), ("rho_fluid", real), ("nu_fluid", real),
+            ("drag_max_per_mass", real),
             ("alpha", real), ("g", real * 3), ("dt", real), ("max_disp", real), ("wall_damping", real)]
 
 

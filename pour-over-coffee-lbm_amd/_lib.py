@@ -36,6 +36,8 @@ POL_KIND_FLUID, POL_KIND_SOLID, POL_KIND_INLET, POL_KIND_OUTLET = 0, 1, 2, 3
 POL_FLAG_POROUS = 0x08000000
 POL_FLAG_NO_LES = 0x10000000
 POL_RFLAG_CLAMP = 0x80000000
+POL_DRAG_TEXTBOOK = 0
+POL_DRAG_REFERENCE = 1
 POL_WANT_MACRO = 0x1
 POL_WANT_REDUCE = 0x2
 RED_SLOTS = ("mass", "mom_x", "mom_y", "mom_z", "kinetic_energy", "max_velocity", "nonfinite_cells",
@@ -63,7 +65,7 @@ class PolBuffers(C.Structure):
 
 class PolParticleDesc(C.Structure):
     _fields_ = [
-        ("count", C.c_int32), ("reserved", C.c_int32),
+        ("count", C.c_int32), ("drag_law", C.c_int32),
         ("rho_fluid", C.c_double), ("nu_fluid", C.c_double), ("drag_max_per_mass", C.c_double),
         ("relax_alpha", C.c_double), ("gravity", C.c_double * 3), ("dt", C.c_double),
         ("max_disp", C.c_double), ("wall_damping", C.c_double),

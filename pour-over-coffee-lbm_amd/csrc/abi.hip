@@ -278,8 +278,11 @@ int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particl
   if (!(d->dt > 0.0) || !(d->max_disp > 0.0)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: dt and max_disp must be > 0");
   ParticleArgs& p = c->pa;
   p = ParticleArgs{};
+  if (d->drag_law != POL_DRAG_TEXTBOOK && d->drag_law != POL_DRAG_REFERENCE)
+    return fail(c, POL_ERR_INVALID, "pol_particles_bind: unknown drag_law");
   p.count = d->count;
   p.real_is_double = c->d.storage == POL_F64;
+  p.drag_law = d->drag_law;
   p.nx = c->d.nx;
   p.ny = c->d.ny;
   p.nz = c->d.nz;

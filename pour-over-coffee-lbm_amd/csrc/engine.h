@@ -41,6 +41,7 @@ hipError_t launch_init_equilibrium(int storage, void* f, const void* rho, const 
 struct ParticleArgs {
   int32_t count;
   int32_t real_is_double;
+  int32_t drag_law;
   uint32_t nx, ny, nz, nxy, N;
   const uint32_t* flags;
   void *pos, *vel, *radius, *mass;

@@ -105,6 +105,7 @@ __global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) 
     cdp[p] = cd;
     const R area = R(3.14159) * r * r;  // the reference's literal
     R dmag = R(0.5) * rho_f * cd * area * mag;
+    if (a.drag_law == POL_DRAG_TEXTBOOK) dmag *= mag;
     dmag = fmin(dmag, ((const R*)a.mass)[p] * (R)a.drag_max_per_mass);
     const R s = dmag / mag;
     dn[0] = s * rx;

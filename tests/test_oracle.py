@@ -1,0 +1,370 @@
+"""Pins of the C oracle (CPU): every checked quantity is fixed by mathematics
+or by the reference's own known-answer data, not by re-running the oracle's
+formula.  If one of these fails the oracle is wrong, whatever the GPU says.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from pourover_amd import _lib as L
+from pourover_amd.geometry import link_flags
+from pourover_amd.lattice import E, W, equilibrium
+
+PER = L.POL_PERIODIC_X | L.POL_PERIODIC_Y | L.POL_PERIODIC_Z
+
+
+def _flags_box(shape, porous=False, periodic=(True, True, True)):
+    flags, _ = link_flags(np.zeros(shape, dtype=np.uint8), periodic=periodic)
+    if porous:
+        flags |= np.uint32(L.POL_FLAG_POROUS | L.POL_FLAG_NO_LES)
+    return flags
+
+
+def test_guo_momentum_law(oracle):
+    """Uniform force F, periodic box, start at rest: the Guo scheme adds exactly F
+    per cell per step to sum f e, and reports rho u = sum f e + F/2."""
+    n, steps, F = 6, 37, np.array([2e-4, -1e-4, 3e-5])
+    d = oracle.make_desc(n, n, n, tau=0.8, options=PER | L.POL_GRAVITY, gravity=tuple(F))
+    f = oracle.init_equilibrium(n, n, n)
+    f, _, _, _ = oracle.run(d, f, steps)
+    _, _, red = oracle.moments(d, f)
+    expect = n**3 * (steps + 0.5) * F
+    assert np.allclose(red[1:4], expect, rtol=1e-11, atol=0)
+    assert abs(red[0] - n**3) < 1e-10
+
+
+def test_mass_momentum_conserved_without_force(oracle):
+    n = 8
+    rng = np.random.default_rng(3)
+    u = 0.03 * rng.standard_normal((3, n, n, n))
+    rho = 1 + 0.01 * rng.standard_normal((n, n, n))
+    d = oracle.make_desc(n, n, n, tau=0.7, options=PER)
+    f = oracle.init_equilibrium(n, n, n, rho, u)
+    _, _, red0 = oracle.moments(d, f)
+    f, _, _, _ = oracle.run(d, f, 50)
+    _, _, red = oracle.moments(d, f)
+    assert abs(red[0] - red0[0]) < 1e-11
+    assert np.allclose(red[1:4], red0[1:4], atol=1e-13)
+
+
+def test_sealed_box_mass_with_walls_and_les(oracle):
+    """Halfway bounce-back on every face (non-periodic box): mass invariant."""
+    n = 10
+    kind = np.zeros((n, n, n), dtype=np.uint8)
+    kind[0, :, :] = kind[-1, :, :] = kind[:, 0, :] = kind[:, -1, :] = kind[:, :, 0] = kind[:, :, -1] = L.POL_KIND_SOLID
+    flags, _ = link_flags(kind, periodic=(False, False, False))
+    rng = np.random.default_rng(4)
+    u = 0.04 * rng.standard_normal((3, n, n, n))
+    d = oracle.make_desc(n, n, n, tau=0.55, options=L.POL_USE_FLAGS | L.POL_LES, cs_smag=0.17)
+    f = oracle.init_equilibrium(n, n, n, None, u)
+    _, _, red0 = oracle.moments(d, f, flags)
+    f, _, _, _ = oracle.run(d, f, 300, flags=flags)
+    _, _, red = oracle.moments(d, f, flags)
+    assert abs(red[0] - red0[0]) / red0[0] < 1e-13
+    assert red[8] == (n - 2) ** 3
+
+
+def test_poiseuille_exact_at_magic_tau(oracle):
+    """Force-driven channel between halfway bounce-back walls: with BGK at
+    tau = 1/2 + sqrt(3/16) the steady profile is the exact parabola
+    u(y) = g/(2 nu) (y - 1/2)(H + 1/2 - y), walls half a cell outside the
+    fluid rows y = 1..H (He, Zou, Luo & Dembo, J. Stat. Phys. 87, 1997)."""
+    H = 8
+    ny = H + 2
+    tau = 0.5 + math.sqrt(3.0 / 16.0)
+    nu = (tau - 0.5) / 3.0
+    g = 1e-6
+    kind = np.zeros((2, ny, 2), dtype=np.uint8)
+    kind[:, 0, :] = kind[:, -1, :] = L.POL_KIND_SOLID
+    flags, _ = link_flags(kind, periodic=(True, False, True))
+    opts = L.POL_PERIODIC_X | L.POL_PERIODIC_Z | L.POL_USE_FLAGS | L.POL_GRAVITY
+    d = oracle.make_desc(2, ny, 2, tau=tau, options=opts, gravity=(g, 0.0, 0.0))
+    f = oracle.init_equilibrium(2, ny, 2)
+    f, _, _, _ = oracle.run(d, f, 3000, flags=flags)
+    _, u, _ = oracle.moments(d, f, flags)
+    y = np.arange(1, H + 1, dtype=np.float64)
+    exact = g / (2 * nu) * (y - 0.5) * (H + 0.5 - y)
+    got = u[0, 0, 1:H + 1, 0]
+    assert np.allclose(got, exact, rtol=1e-9, atol=0)
+    assert np.abs(u[1:]).max() < 1e-15
+
+
+@pytest.mark.parametrize("K,CF,G", [(1.0, 0.55, 1e-3), (1e-3, 0.55, 1e-5), (1e-2, 2.0, 5e-3)])
+def test_porous_terminal_velocity(oracle, K, CF, G):
+    """Uniform force in an all-porous periodic box: the steady velocity solves
+    G = rho (nu/K) u + rho (C_F/sqrt K) u^2 (Darcy-Forchheimer balance)."""
+    n, tau = 4, 0.6
+    nu = (tau - 0.5) / 3.0
+    flags = _flags_box((n, n, n), porous=True)
+    d = oracle.make_desc(n, n, n, tau=tau, options=PER | L.POL_USE_FLAGS | L.POL_GRAVITY, gravity=(0, 0, -G),
+                         darcy_K=K, forch_CF=CF)
+    f = oracle.init_equilibrium(n, n, n)
+    f, _, _, _ = oracle.run(d, f, 3000, flags=flags)
+    rho, u, _ = oracle.moments(d, f, flags)
+    a, b = nu / K, CF / math.sqrt(K)
+    ut = (-a + math.sqrt(a * a + 4 * b * G)) / (2 * b)
+    assert np.allclose(rho, 1.0, atol=1e-13)
+    assert np.allclose(u[2], -ut, rtol=1e-10)
+    assert np.abs(u[:2]).max() < 1e-15
+
+
+def _shear_state(rho, tau_t, gamma):
+    """feq(rho, 0) + non-equilibrium part with stress Pi = -2 rho c_s^2 tau_t S,
+    S_xz = S_zx = gamma / 2 (Hermite projection w/(2 c_s^4) (e e - c_s^2 I):Pi)."""
+    cs2 = 1.0 / 3.0
+    Pi = np.zeros((3, 3))
+    Pi[0, 2] = Pi[2, 0] = -2 * rho * cs2 * tau_t * gamma / 2
+    Ef = E.astype(np.float64)
+    Qt = np.einsum("qa,qb->qab", Ef, Ef) - cs2 * np.eye(3)[None]
+    f1 = W / (2 * cs2 * cs2) * np.einsum("qab,ab->q", Qt, Pi)
+    feq = equilibrium(np.array([rho]), np.zeros((3, 1)))[:, 0]
+    return feq, f1
+
+
+@pytest.mark.parametrize("gamma", [0.0, 1e-3, 2e-2, 0.5])
+def test_les_uniform_shear_closed_form(oracle, gamma):
+    """Smagorinsky: uniform shear |S| = gamma gives nu_t = (C_s Delta)^2 gamma
+    (Delta = 1), tau = tau0 + 3 nu_t, capped at nu_t <= 0.1 (les_turbulence.py:375).
+    A 1x1x1 periodic box makes one step a pure collision of the built state."""
+    tau0, cs, rho = 0.6, 0.17, 1.02
+    nu_t = min(cs * cs * gamma, 0.1)
+    tau_t = tau0 + 3 * nu_t
+    feq, f1 = _shear_state(rho, tau_t, gamma)
+    f = (feq + f1).reshape(19, 1, 1, 1)
+    d = oracle.make_desc(1, 1, 1, tau=tau0, options=PER | L.POL_LES, cs_smag=cs, nu_sgs_max=0.1)
+    fo, _, _, _ = oracle.step(d, f)
+    if cs * cs * gamma <= 0.1:
+        expect = feq + (1 - 1 / tau_t) * f1
+        assert np.allclose(fo[:, 0, 0, 0], expect, rtol=1e-12, atol=1e-17)
+    else:  # clamp: relaxation with tau0 + 0.3 while the state encodes a larger tau
+        expect = feq + (1 - 1 / (tau0 + 0.3)) * f1
+        assert np.allclose(fo[:, 0, 0, 0], expect, rtol=1e-12, atol=1e-17)
+
+
+def test_les_off_in_masked_cells(oracle):
+    """LES mask = 0 (filter zone, filter_paper.py:199-204) keeps tau0 (reference test :112-130)."""
+    tau0 = 0.6
+    feq, f1 = _shear_state(1.0, 0.9, 0.05)
+    f = (feq + f1).reshape(19, 1, 1, 1)
+    flags = np.full((1, 1, 1), L.POL_FLAG_NO_LES, dtype=np.uint32)
+    d = oracle.make_desc(1, 1, 1, tau=tau0, options=PER | L.POL_LES | L.POL_USE_FLAGS, cs_smag=0.17)
+    fo, _, _, _ = oracle.step(d, f, flags)
+    assert np.allclose(fo[:, 0, 0, 0], feq + (1 - 1 / tau0) * f1, rtol=1e-12, atol=1e-17)
+
+
+def test_streaming_moves_each_population_one_link(oracle):
+    """tau = 1 makes f* = feq(moments): a single-population excess at cell c
+    shows up only at c + e_q in the next gather."""
+    n = 5
+    for q in (1, 8, 13, 18):
+        f = oracle.init_equilibrium(n, n, n)
+        f[q, 2, 2, 2] += 1e-3
+        d = oracle.make_desc(n, n, n, tau=1.0, options=PER)
+        rho, _, _ = oracle.moments(d, f)
+        ex, ey, ez = E[q]
+        hot = np.argwhere(np.abs(rho - 1.0) > 1e-12)
+        assert hot.tolist() == [[(2 + ez) % n, (2 + ey) % n, (2 + ex) % n]]
+
+
+def test_fp32_oracle_close_to_fp64_on_v60(oracle):
+    import pourover_amd.workloads as WL
+
+    wl = WL.v60_config(32, "f64")
+    flags = wl.geometry.flags
+    d = oracle.desc_from_dict(wl.config.oracle_dict(True))
+    f64 = oracle.init_equilibrium(32, 32, 32)
+    f32 = oracle.init_equilibrium(32, 32, 32, prec="f32")
+    f64, _, _, _ = oracle.run(d, f64, 40, flags=flags)
+    f32, _, _, _ = oracle.run(d, f32, 40, flags=flags, prec="f32")
+    r64, u64, _ = oracle.moments(d, f64, flags)
+    r32, u32, _ = oracle.moments(d, f32, flags, prec="f32")
+    assert rel_l2(u32, u64) < 1e-4
+    assert rel_l2(r32, r64) < 1e-5
+
+
+# ---- CPUBackend restatement (cpu_backend.py:79-190) --------------------------
+def test_cpu_backend_restatement_interior_matches_periodic(oracle):
+    """Away from the box edges the CPUBackend step (collide + pull stream, edges
+    keep their own value) is the periodic BGK step; after t steps cells farther
+    than t from every face agree (ordering differs; compare moments)."""
+    n, t = 16, 3
+    rng = np.random.default_rng(5)
+    u = 0.02 * rng.standard_normal((3, n, n, n))
+    # CPUBackend's own direction order (cpu_backend.py:69-71)
+    cb = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
+                   [-1, 1, 0], [1, -1, 0], [-1, -1, 0], [0, 1, 1], [0, -1, 1], [1, 0, 1], [-1, 0, 1], [1, 0, -1],
+                   [-1, 0, -1], [0, 1, -1], [0, -1, -1]])
+    perm = [int(np.nonzero((E == e).all(axis=1))[0][0]) for e in cb]
+    s0 = oracle.init_equilibrium(n, n, n, None, u)     # pre-collision state S_0
+    # CPUBackend: f holds S_k; a call collides then streams: S_{k+1} = Stream(C(S_k)),
+    # and reports rho/u = moments of S_k.
+    f_cb = np.ascontiguousarray(s0[perm].astype(np.float32))
+    f_new = np.zeros_like(f_cb)
+    rho = np.zeros((n, n, n), np.float32)
+    uu = np.zeros((3, n, n, n), np.float32)
+    for _ in range(t):
+        oracle.cpu_backend_step(f_cb, f_new, rho, uu, None, 0.6)
+    # pull oracle: P_0 = C(S_0), P_{k+1} = C(Gather(P_k)); Gather(P_k) = S_{k+1}
+    # in the interior, so moments(P_{t-2}) = moments(S_{t-1}) = the t-th report.
+    d = oracle.make_desc(n, n, n, tau=0.6, options=PER)
+    p = _collide_only(oracle, s0, 0.6)
+    for _ in range(t - 2):
+        p, _, _, _ = oracle.step(d, p)
+    rho_p, u_p, _ = oracle.moments(d, p)
+    s = slice(t + 1, n - t - 1)
+    assert np.allclose(rho[s, s, s], rho_p[s, s, s], atol=2e-6)
+    assert np.allclose(uu[:, s, s, s], u_p[:, s, s, s], atol=2e-7)
+    # ... and the box edges differ (no periodic wrap in the reference's CPU backend)
+    assert not np.allclose(uu[:, 0, :, :], u_p[:, 0, :, :], atol=2e-7)
+
+
+def _collide_only(oracle, f, tau):
+    """Collision of every cell with no streaming: a 1x1x1 periodic step per cell."""
+    out = np.empty_like(f)
+    d = oracle.make_desc(1, 1, 1, tau=tau, options=PER)
+    nz, ny, nx = f.shape[1:]
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                fo, _, _, _ = oracle.step(d, f[:, k:k + 1, j:j + 1, i:i + 1].copy())
+                out[:, k, j, i] = fo[:, 0, 0, 0]
+    return out
+
+
+def test_cpu_backend_clamp_and_solid_pass(oracle):
+    """f >= 0 clamp (cpu_backend.py:132-133) and the destructive solid pass
+    (:172-183: sequential f[opp[q]] = f[q] with the hard-coded table)."""
+    n = 3
+    f = np.zeros((19, n, n, n), np.float32)
+    f[0] = 1.0
+    f[1, 1, 1, 1] = 0.5           # moving +x: some equilibria below the state -> clamp path
+    f_new = np.zeros_like(f)
+    rho = np.zeros((n, n, n), np.float32)
+    uu = np.zeros((3, n, n, n), np.float32)
+    solid = np.zeros((n, n, n), np.uint8)
+    solid[0, 0, 0] = 1
+    f[:, 0, 0, 0] = np.arange(19, dtype=np.float32)
+    oracle.cpu_backend_step(f, f_new, rho, uu, solid, 0.51)
+    assert (f_new[:, 1:, 1:, 1:] >= 0).all()
+    # solid cell: after streaming it pulled neighbours' values (edges keep own),
+    # then pairs (1,2), (3,4), ... all end equal to the odd member
+    for a, b in ((1, 2), (3, 4), (5, 6), (7, 8), (9, 10), (11, 12), (13, 14), (15, 16), (17, 18)):
+        assert f[a, 0, 0, 0] == f[b, 0, 0, 0]
+
+
+# ---- particles (coffee_particles.py) ---------------------------------------------
+def test_particle_index_edge_cases(oracle):
+    """i = (int)max(0, min(n-2, p)) in float32, frac = clamp(p - i, 0, 1)."""
+    n = 16
+    one = np.float32(1)
+    vals = [0.0, 0.5, 1.0, 3.0, np.nextafter(np.float32(3), np.float32(0)), np.nextafter(np.float32(3), np.float32(9)),
+            13.999999, 14.0, 14.5, 15.0, 15.999, 16.0, 40.0, -0.0, -1e-8, -3.0, np.inf, -np.inf]
+    pos = np.tile(np.array(vals, dtype=np.float32), (3, 1))
+    idx, frac = oracle.particle_index(pos, (n, n, n))
+    exp_i = []
+    for v in np.array(vals, dtype=np.float32):
+        c = min(max(float(v), 0.0), 14.0)
+        exp_i.append(int(math.floor(c)))
+    assert idx[0].tolist() == exp_i
+    for v, i, fr in zip(np.array(vals, dtype=np.float32), idx[0], frac[0]):
+        if np.isfinite(v):
+            assert fr == np.float32(min(max(np.float32(v) - np.float32(i), 0.0), 1.0))
+        else:
+            assert fr == (one if v > 0 else 0)
+    # 3 - ulp stays in cell 2 with a fraction just below 1
+    k = vals.index(np.nextafter(np.float32(3), np.float32(0)))
+    assert idx[0][k] == 2 and 0.999 < frac[0][k] < 1.0
+
+
+def test_trilinear_reference_known_answers(oracle):
+    """Linear field v = (x, y, z) on 16^3 reproduces each of the reference's ten
+    probe positions (tests/test_trilinear_interpolation.py:143-166)."""
+    import json
+    import os
+
+    kat = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))["trilinear_kat"]
+    n = kat["grid"]
+    pos = np.array(kat["positions"], dtype=np.float64).T.copy()
+    P = pos.shape[1]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    u_field = np.stack([x, y, z]).astype(np.float64)
+    st = {"pos": pos, "vel": u_field[:, 0, 0, 0:1].repeat(P, axis=1) * 0 + pos, "radius": np.full(P, 0.5),
+          "mass": np.ones(P), "active": np.ones(P, np.int32), "drag_old": np.zeros((3, P))}
+    pp = oracle.particle_params(P, rho_fluid=1.0, nu_fluid=0.1, drag_max_per_mass=1e9, alpha=0.8, g=(0, 0, 0),
+                                dt=1.0, max_disp=1.0, wall_damping=0.3)
+    force = np.zeros((3, n, n, n))
+    oracle.particles_couple((n, n, n), pp, st, u_field, force)
+    assert np.allclose(st["u_fluid"], np.array(kat["expected"]).T, atol=1e-12)
+
+
+@pytest.mark.parametrize("re,cd", [
+    (0.05, 24 / 0.05), (0.001, 24 / 0.01), (0.0999, 24 / 0.0999), (0.1, 24 / 0.1 * (1 + 0.15 * 0.1**0.687)),
+    (1.0, 24 * 1.15), (10.0, 2.4 * (1 + 0.15 * 10**0.687)), (999.9, 24 / 999.9 * (1 + 0.15 * 999.9**0.687)),
+    (1000.0, 0.44), (5000.0, 0.44)])
+def test_schiller_naumann_piecewise(oracle, re, cd):
+    assert abs(oracle.drag_coefficient(re) - cd) <= 1e-12 * cd
+
+
+def _one_particle(oracle, n=8, r=0.75, m=1.0, vel=(0.0, 0.0, 0.0), u=(0.0, 0.0, 0.0), alpha=0.8, g=(0, 0, 0)):
+    P = 1
+    pos = np.array([[3.3], [3.6], [4.2]])
+    st = {"pos": pos.copy(), "vel": np.array(vel, dtype=np.float64).reshape(3, 1), "radius": np.array([r]),
+          "mass": np.array([m]), "active": np.ones(P, np.int32), "drag_old": np.zeros((3, P))}
+    u_field = np.zeros((3, n, n, n))
+    for i in range(3):
+        u_field[i] = u[i]
+    pp = oracle.particle_params(P, rho_fluid=1.0, nu_fluid=1 / 30, drag_max_per_mass=1e9, alpha=alpha, g=g, dt=1.0,
+                                max_disp=1.0, wall_damping=0.3)
+    return st, u_field, pp
+
+
+@pytest.mark.parametrize("law", [L.POL_DRAG_TEXTBOOK, L.POL_DRAG_REFERENCE])
+def test_drag_closed_form_and_newton_third_law(oracle, law):
+    """|F| = 1/2 rho C_D pi r^2 |u_rel|^2 (textbook) or ... |u_rel| (reference as coded,
+    coffee_particles.py:1137-1145), direction u_rel; the reaction spread to the 8
+    corners sums to -F (partition of unity); under-relaxation alpha F_new + (1-alpha) F_old."""
+    st, uf, pp = _one_particle(oracle, u=(0.01, -0.02, 0.005))
+    pp.drag_law = law
+    force = np.zeros((3, 8, 8, 8))
+    oracle.particles_couple((8, 8, 8), pp, st, uf, force)
+    rel = np.array([0.01, -0.02, 0.005])
+    mag = np.linalg.norm(rel)
+    re = mag * 2 * 0.75 / (1 / 30)
+    cd = 24 / re * (1 + 0.15 * re**0.687) if re >= 0.1 else 24 / max(re, 0.01)
+    drag = 0.5 * cd * 3.14159 * 0.75**2 * mag * rel / mag
+    if law == L.POL_DRAG_TEXTBOOK:
+        drag = drag * mag
+    assert np.allclose(st["drag_new"][:, 0], drag, rtol=1e-12)
+    assert np.allclose(force.reshape(3, -1).sum(axis=1), -drag, rtol=1e-12)   # partition of unity + reaction
+    assert np.allclose(st["drag"][:, 0], 0.8 * drag, rtol=1e-12)              # alpha F_new + (1-alpha) 0
+    oracle.particles_couple((8, 8, 8), pp, st, uf, force)
+    assert np.allclose(st["drag"][:, 0], 0.8 * drag + 0.2 * 0.8 * drag, rtol=1e-12)
+
+
+def test_settling_velocity(oracle):
+    """Quiescent fluid, buoyancy-corrected gravity g: the particle reaches the
+    terminal speed where Schiller-Naumann drag balances m g."""
+    r, m, g = 0.75, 1.2 * math.pi * 1.5**3 / 6, 2e-5
+    st, uf, pp = _one_particle(oracle, r=r, m=m, g=(0, 0, -g))
+    st["pos"][2] = 6.0
+    force = np.zeros((3, 8, 8, 8))
+    for _ in range(3000):
+        st["pos"][:, 0] = [3.3, 3.6, 4.2]      # keep it in place: the fluid field is uniform
+        oracle.particles_couple((8, 8, 8), pp, st, uf, force)
+        oracle.particles_advance((8, 8, 8), pp, st, None, force)
+    v = -st["vel"][2, 0]
+
+    def drag(vv):
+        re = vv * 2 * r / (1 / 30)
+        cd = 24 / re * (1 + 0.15 * re**0.687) if re >= 0.1 else 24 / max(re, 0.01)
+        return 0.5 * cd * 3.14159 * r * r * vv * vv
+
+    lo, hi = 1e-12, 1.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if drag(mid) < m * g else (lo, mid)
+    assert abs(v - lo) / lo < 1e-9
