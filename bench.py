@@ -85,6 +85,28 @@ def bench_periodic(precision, steps, warmup, n=224):
     return {"event_s": ev_s, "wall_s": wall, "cells": n**3}
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """torch device-to-device copy of `nbytes` (read nbytes + write nbytes per launch):
+    the reference point for rocprof FETCH_SIZE / WRITE_SIZE and for the
+    practical bandwidth ceiling."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    stop.record()
+    torch.cuda.synchronize()
+    s = start.elapsed_time(stop) / 1e3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"bytes_per_copy": nbytes, "copy_ms": round(s * 1e3, 4), "copy_GBs": round(2 * nbytes / s / 1e9, 1)}
+
+
 def cpu_baseline(budget_s=12.0):
     """The oracle's C restatement (OpenMP, all host threads) on the same V60 224^3
     workload for a bounded number of steps."""
@@ -119,6 +141,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only (no secondary lines, no CPU baseline)")
     ap.add_argument("--precision", default="f32")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the multi-GPU (z-slab halo exchange) path is not part of this build",
@@ -132,6 +156,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     prec = args.precision
+    if args.calibrate_copy:
+        out = copy_calibration()
+        log("bench: copy calibration %s" % json.dumps(out))
     log("bench: v60 224^3 %s, %d warmup + %d timed steps" % (prec, args.warmup, args.steps))
     r = bench_v60(prec, args.steps, args.warmup)
     k_s = r["event_s"] / args.steps

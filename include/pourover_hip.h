@@ -120,7 +120,16 @@ typedef struct pol_buffers {
   void *perm;           /* real [cells] permeability (POL_PERM_FIELD) */
   void *rho;            /* real [cells] macroscopic output, may be NULL */
   void *u;              /* real [3][cells] macroscopic output, may be NULL */
+  const uint32_t *active_chunks; /* ascending first-cell indices of the
+                                    POL_CHUNK-cell row chunks (x0 % POL_CHUNK
+                                    == 0) that hold at least one non-solid
+                                    cell, or NULL to launch over every cell;
+                                    with a list the kernels touch only those
+                                    chunks (solid lanes idle) */
+  uint32_t n_chunks;
+  uint32_t reserved;
 } pol_buffers;
+#define POL_CHUNK 32 /* one 128-byte line of FP32 populations */
 
 /* ---- library ------------------------------------------------------------ */
 /* ABI version (POL_ABI_VERSION). */

@@ -348,7 +348,7 @@ void SFX(oracle_particle_index)(const float *pos, int n, int nx, int ny, int nz,
       int i = (int)c;
       float t = pp - (float)i;
       idx[(size_t)d * n + p] = i;
-      frac[(size_t)d * n + p] = fmaxf(0.0f, fminf(1.0f, t));
+      frac[(size_t)d * n + p] = fmaxf(0.0f, fminf(1.0f, t)) + 0.0f; /* -0 -> +0 */
     }
 }
 

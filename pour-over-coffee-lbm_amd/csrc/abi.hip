@@ -16,7 +16,8 @@ struct pol_ctx {
   bool bound = false;
   int cur = 0;
   uint32_t N = 0;
-  Grid grid{};
+  Grid grid{};   // dense grid over all cells (sizes the partial buffers)
+  Grid wgrid{};  // work grid: over the active-cell list when bound, else = grid
   double* partial = nullptr;  // [nblk][POL_NRED]
   double* red_dev = nullptr;  // [POL_NRED]
   // particles
@@ -96,6 +97,7 @@ int pol_create(pol_ctx** out, const pol_desc* desc) {
   c->d = d;
   c->N = (uint32_t)N;
   c->grid = make_grid(N);
+  c->wgrid = c->grid;
   hipError_t e = hipSetDevice(d.device);
   if (e == hipSuccess) e = hipMalloc((void**)&c->partial, (size_t)c->grid.nblk * POL_NRED * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->red_dev, POL_NRED * sizeof(double));
@@ -134,8 +136,13 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
   if ((o & POL_FORCE_FIELD) && !b->force) return fail(c, POL_ERR_INVALID, "pol_bind: POL_FORCE_FIELD without force buffer");
   if ((o & POL_PERM_FIELD) && !b->perm) return fail(c, POL_ERR_INVALID, "pol_bind: POL_PERM_FIELD without perm buffer");
   if ((o & POL_PERM_FIELD) && !(o & POL_USE_FLAGS)) return fail(c, POL_ERR_INVALID, "pol_bind: permeability needs cell flags");
+  if (b->active_chunks && (b->n_chunks == 0 || (uint64_t)b->n_chunks * POL_CHUNK > (uint64_t)c->N + POL_CHUNK))
+    return fail(c, POL_ERR_INVALID, "pol_bind: n_chunks out of range");
+  if (b->active_chunks && !(o & POL_USE_FLAGS))
+    return fail(c, POL_ERR_INVALID, "pol_bind: a chunk list needs cell flags (solid lanes are masked by them)");
   c->b = *b;
   if (!(o & POL_USE_FLAGS)) c->b.cell_flags = nullptr;
+  c->wgrid = c->b.active_chunks ? make_grid((uint64_t)c->b.n_chunks * POL_CHUNK) : c->grid;
   c->bound = true;
   c->cur = 0;
   c->err.clear();
@@ -180,8 +187,10 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.nxy = (uint32_t)d.nx * d.ny;
   a.dnx = make_fastdiv(d.nx);
   a.dny = make_fastdiv(d.ny);
-  a.per_xcd = c->grid.per_xcd;
-  a.nblk = c->grid.nblk;
+  a.chunks = c->b.active_chunks;
+  a.n_items = c->b.active_chunks ? c->b.n_chunks * POL_CHUNK : c->N;
+  a.per_xcd = c->wgrid.per_xcd;
+  a.nblk = c->wgrid.nblk;
   a.options = d.options;
   a.what = what;
   a.tau0 = d.tau;
@@ -205,7 +214,7 @@ static int check_what(pol_ctx* c, uint32_t what, double* red_out, const char* fn
 }
 
 static int finish_reduce(pol_ctx* c, double* red_out, hipStream_t s, const char* fn) {
-  hipError_t e = launch_reduce_final(c->partial, c->grid.nblk, c->red_dev, s);
+  hipError_t e = launch_reduce_final(c->partial, c->wgrid.nblk, c->red_dev, s);
   if (e == hipSuccess) e = hipMemcpyAsync(red_out, c->red_dev, POL_NRED * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, fn);
@@ -222,7 +231,7 @@ int pol_step_ex(pol_ctx* c, int nsteps, uint32_t what, double* red_out, void* st
   for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
     const uint32_t w = (i == nsteps - 1) ? what : 0u;
     StepArgs a = make_args(c, c->b.f[c->cur], c->b.f[1 - c->cur], w);
-    e = launch_stream_collide(c->d.storage, a, true, c->grid, s);
+    e = launch_stream_collide(c->d.storage, a, true, c->wgrid, s);
     if (e == hipSuccess) c->cur = 1 - c->cur;
   }
   if (e != hipSuccess) return hip_fail(c, e, "pol_step");
@@ -240,7 +249,7 @@ int pol_step_from(pol_ctx* c, const void* fin, void* fout, uint32_t what, double
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(c->d.device);
   StepArgs a = make_args(c, fin, fout, what);
-  if (e == hipSuccess) e = launch_stream_collide(c->d.storage, a, true, c->grid, s);
+  if (e == hipSuccess) e = launch_stream_collide(c->d.storage, a, true, c->wgrid, s);
   if (e != hipSuccess) return hip_fail(c, e, "pol_step_from");
   if (what & POL_WANT_REDUCE) return finish_reduce(c, red_out, s, "pol_step_from");
   return POL_OK;
@@ -255,7 +264,7 @@ int pol_macroscopic(pol_ctx* c, uint32_t what, double* red_out, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(c->d.device);
   StepArgs a = make_args(c, c->b.f[c->cur], nullptr, what);
-  if (e == hipSuccess) e = launch_stream_collide(c->d.storage, a, false, c->grid, s);
+  if (e == hipSuccess) e = launch_stream_collide(c->d.storage, a, false, c->wgrid, s);
   if (e != hipSuccess) return hip_fail(c, e, "pol_macroscopic");
   if (what & POL_WANT_REDUCE) return finish_reduce(c, red_out, s, "pol_macroscopic");
   return POL_OK;
@@ -379,8 +388,8 @@ int pol_adjoint_step(pol_ctx* c, const void* f_state, int objective, void* strea
   A.partial = c->adj_partial;
   A.objective = objective;
   hipError_t e = hipSetDevice(c->d.device);
-  if (e == hipSuccess) e = launch_adjoint(c->d.storage, A, c->ab.rev_flags, c->grid, s);
-  if (e == hipSuccess) e = launch_adjoint_final(c->adj_partial, c->grid.nblk, c->adj_acc, s);
+  if (e == hipSuccess) e = launch_adjoint(c->d.storage, A, c->ab.rev_flags, c->wgrid, s);
+  if (e == hipSuccess) e = launch_adjoint_final(c->adj_partial, c->wgrid.nblk, c->adj_acc, s);
   if (e != hipSuccess) return hip_fail(c, e, "pol_adjoint_step");
   c->acur = 1 - c->acur;
   return POL_OK;

@@ -59,12 +59,9 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   const StepArgs& a = A.fw;
   const uint32_t lblk = logical_block(a.per_xcd);
   if (lblk >= a.nblk) return;
-  const uint32_t idx = lblk * BLOCK + threadIdx.x;
-  const bool inside = idx < a.N;
-  const uint32_t cell = inside ? idx : 0u;
+  uint32_t cell, yz, x;
+  const bool inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.dnx, a.nx, cell, yz, x);
   const uint32_t N = a.N;
-  const uint32_t yz = fdiv(cell, a.dnx);
-  const uint32_t x = cell - yz * a.nx;
   const uint32_t z = fdiv(yz, a.dny);
   const uint32_t y = yz - z * a.ny;
   const uint32_t fl = inside ? a.flags[cell] : (POL_KIND_SOLID << POL_KIND_SHIFT);

@@ -89,42 +89,70 @@ struct CellOut {
   R ux, uy, uz;
 };
 
+// Opposite-direction pairs (a, opp a): e_opp = -e, w_opp = w.  The nine
+// first members in config/core.py order; the rest population q = 0 is alone.
+__host__ __device__ constexpr int PAIR_A(int p) {
+  constexpr int t[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+  return t[p];
+}
+
+// d = 36 (feq(rho, u) - w) written pairwise: for e_b = -e_a
+//   36 (feq_a - w) = sym + anti, 36 (feq_b - w) = sym - anti,
+//   sym = c (rd + rho (4.5 (e.u)^2 - 1.5 u.u)), anti = 3 c rho e.u
+template <typename R>
+__device__ inline void set_equilibrium(R (&d)[Q], R rd, R rho, R ux, R uy, R uz) {
+  const R usq = ux * ux + uy * uy + uz * uz;
+  const R base = rd - R(1.5) * rho * usq;
+  d[0] = CW<R>(0) * base;
+  static_for<0, 9>([&](auto pc) {
+    constexpr int a = PAIR_A(decltype(pc)::value);
+    constexpr int b = OPP(a);
+    const R eu = R(EX(a)) * ux + R(EY(a)) * uy + R(EZ(a)) * uz;
+    const R sym = CW<R>(a) * (base + R(4.5) * rho * eu * eu);
+    const R anti = R(3) * CW<R>(a) * rho * eu;
+    d[a] = sym + anti;
+    d[b] = sym - anti;
+  });
+}
+
 // Collide one gathered cell in place: d[] in = pre-collision scaled
 // deviations D = 36 (f - w), out = post-collision ones.  kind/porous from the
 // cell flags; K = permeability, (fx,fy,fz) = extra force density.
+//
+// Evaluated pairwise over opposite directions: equilibrium and Guo source of
+// a pair share their symmetric part and negate the antisymmetric one, and the
+// non-equilibrium stress needs only the pair sums (e_b e_b = e_a e_a), which
+// halves the arithmetic of a per-direction evaluation.
 template <typename R, bool FULL>
 __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous, bool les, R K, R fx, R fy, R fz,
                                           const Phys<R>& P) {
   CellOut<R> o;
   if (FULL && kind == POL_KIND_INLET) {
-    const R rho = P.inlet_rho, rd = P.inlet_rho - R(1);
-    o.rd = rd;
+    o.rd = P.inlet_rho - R(1);
     o.ux = P.inlet_u[0];
     o.uy = P.inlet_u[1];
     o.uz = P.inlet_u[2];
-    const R usq = o.ux * o.ux + o.uy * o.uy + o.uz * o.uz;
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      d[q] = deq_q<R>(EX(q), EY(q), EZ(q), CW<R>(q), rd, rho, o.ux, o.uy, o.uz, usq);
-    });
+    set_equilibrium<R>(d, o.rd, P.inlet_rho, o.ux, o.uy, o.uz);
     return o;
   }
-  R s36 = R(0), mx = R(0), my = R(0), mz = R(0);  // 36 (rho - 1), 36 m
-  static_for<0, Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    s36 += d[q];
-    if constexpr (EX(q) > 0) mx += d[q];
-    if constexpr (EX(q) < 0) mx -= d[q];
-    if constexpr (EY(q) > 0) my += d[q];
-    if constexpr (EY(q) < 0) my -= d[q];
-    if constexpr (EZ(q) > 0) mz += d[q];
-    if constexpr (EZ(q) < 0) mz -= d[q];
+  // moments: 36 (rho - 1) = sum d, 36 m = sum_pairs (d_a - d_b) e_a
+  R s36 = d[0], mx = R(0), my = R(0), mz = R(0);
+  static_for<0, 9>([&](auto pc) {
+    constexpr int a = PAIR_A(decltype(pc)::value);
+    constexpr int b = OPP(a);
+    s36 += d[a] + d[b];
+    const R A = d[a] - d[b];
+    if constexpr (EX(a) > 0) mx += A;
+    if constexpr (EX(a) < 0) mx -= A;
+    if constexpr (EY(a) > 0) my += A;
+    if constexpr (EY(a) < 0) my -= A;
+    if constexpr (EZ(a) > 0) mz += A;
+    if constexpr (EZ(a) < 0) mz -= A;
   });
   const R rd = s36 / R(36);
   const R rho = R(1) + rd;
   const R rho36 = R(36) + s36;
   const R inv36 = rho36 > R(36e-12) ? R(1) / rho36 : R(0);  // 1 / (36 rho)
-  const R inv_rho = R(36) * inv36;
   R Fx = R(0), Fy = R(0), Fz = R(0);
   if constexpr (FULL) {
     Fx = P.G[0] + fx;
@@ -150,36 +178,32 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
   o.ux = ux;
   o.uy = uy;
   o.uz = uz;
-  const R usq = ux * ux + uy * uy + uz * uz;
   if (FULL && kind == POL_KIND_OUTLET) {
-    const R ro = P.outlet_rho, rdo = P.outlet_rho - R(1);
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      d[q] = deq_q<R>(EX(q), EY(q), EZ(q), CW<R>(q), rdo, ro, ux, uy, uz, usq);
-    });
+    set_equilibrium<R>(d, P.outlet_rho - R(1), P.outlet_rho, ux, uy, uz);
     return o;
   }
-  R de[Q];
-  static_for<0, Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    de[q] = deq_q<R>(EX(q), EY(q), EZ(q), CW<R>(q), rd, rho, ux, uy, uz, usq);
-  });
+  const R usq = ux * ux + uy * uy + uz * uz;
+  const R base = rd - R(1.5) * rho * usq;  // shared part of every equilibrium
+  const R d0eq = CW<R>(0) * base;
   R tau = P.tau0;
   if (FULL && les && !porous) {
-    // 36 x non-equilibrium stress: Pi_ab = sum_q e_a e_b (f_q - feq_q)
+    // 36 x non-equilibrium stress Pi_ab = sum_q e_a e_b (f_q - feq_q); each pair
+    // contributes e_a e_a^T (n_a + n_b) with n_a + n_b = (d_a + d_b) - 2 sym
     R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
-    static_for<1, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const R n = d[q] - de[q];
-      if constexpr (EX(q) != 0) pxx += n;
-      if constexpr (EY(q) != 0) pyy += n;
-      if constexpr (EZ(q) != 0) pzz += n;
-      if constexpr (EX(q) * EY(q) > 0) pxy += n;
-      if constexpr (EX(q) * EY(q) < 0) pxy -= n;
-      if constexpr (EX(q) * EZ(q) > 0) pxz += n;
-      if constexpr (EX(q) * EZ(q) < 0) pxz -= n;
-      if constexpr (EY(q) * EZ(q) > 0) pyz += n;
-      if constexpr (EY(q) * EZ(q) < 0) pyz -= n;
+    static_for<0, 9>([&](auto pc) {
+      constexpr int a = PAIR_A(decltype(pc)::value);
+      constexpr int b = OPP(a);
+      const R eu = R(EX(a)) * ux + R(EY(a)) * uy + R(EZ(a)) * uz;
+      const R n = (d[a] + d[b]) - R(2) * CW<R>(a) * (base + R(4.5) * rho * eu * eu);
+      if constexpr (EX(a) != 0) pxx += n;
+      if constexpr (EY(a) != 0) pyy += n;
+      if constexpr (EZ(a) != 0) pzz += n;
+      if constexpr (EX(a) * EY(a) > 0) pxy += n;
+      if constexpr (EX(a) * EY(a) < 0) pxy -= n;
+      if constexpr (EX(a) * EZ(a) > 0) pxz += n;
+      if constexpr (EX(a) * EZ(a) < 0) pxz -= n;
+      if constexpr (EY(a) * EZ(a) > 0) pyz += n;
+      if constexpr (EY(a) * EZ(a) < 0) pyz -= n;
     });
     const R q36 = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
     // tau = (tau0 + sqrt(tau0^2 + 18 sqrt2 Cs^2 Q / rho)) / 2 with Q / rho = q36 / (36 rho)
@@ -187,22 +211,27 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
     tau = tau < P.tau_max ? tau : P.tau_max;
   }
   const R omega = R(1) / tau;
-  if (FULL) {
-    const R cf = R(1) - R(0.5) * omega;
-    const R uF = ux * Fx + uy * Fy + uz * Fz;
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const R eu = R(EX(q)) * ux + R(EY(q)) * uy + R(EZ(q)) * uz;
-      const R eF = R(EX(q)) * Fx + R(EY(q)) * Fy + R(EZ(q)) * Fz;
-      const R src = cf * CW<R>(q) * (R(3) * (eF - uF) + R(9) * eu * eF);
-      d[q] = d[q] - omega * (d[q] - de[q]) + src;
-    });
-  } else {
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      d[q] = d[q] - omega * (d[q] - de[q]);
-    });
-  }
+  const R keep = R(1) - omega;
+  // BGK + Guo: d* = (1 - w) d + w deq + S with
+  //   S_q = (1 - w/2) c_q [3 (e_q - u).F + 9 (e_q.u)(e_q.F)]
+  const R cf = FULL ? (R(1) - R(0.5) * omega) : R(0);
+  const R uF = FULL ? (ux * Fx + uy * Fy + uz * Fz) : R(0);
+  d[0] = keep * d[0] + omega * d0eq - (FULL ? R(3) * cf * CW<R>(0) * uF : R(0));
+  static_for<0, 9>([&](auto pc) {
+    constexpr int a = PAIR_A(decltype(pc)::value);
+    constexpr int b = OPP(a);
+    constexpr R c = CW<R>(a);
+    const R eu = R(EX(a)) * ux + R(EY(a)) * uy + R(EZ(a)) * uz;
+    R sym = omega * c * (base + R(4.5) * rho * eu * eu);
+    R anti = omega * R(3) * c * rho * eu;
+    if constexpr (FULL) {
+      const R eF = R(EX(a)) * Fx + R(EY(a)) * Fy + R(EZ(a)) * Fz;
+      sym += cf * c * (R(9) * eu * eF - R(3) * uF);
+      anti += R(3) * cf * c * eF;
+    }
+    d[a] = keep * d[a] + sym + anti;
+    d[b] = keep * d[b] + sym - anti;
+  });
   return o;
 }
 

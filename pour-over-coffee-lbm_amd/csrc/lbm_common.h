@@ -141,6 +141,25 @@ __device__ inline uint32_t logical_block(uint32_t per_xcd) {
   return (b % NXCD) * per_xcd + b / NXCD;
 }
 
+// Work item -> cell.  Dense launches map item = cell.  Chunk launches map
+// item to lane (item % POL_CHUNK) of row chunk chunks[item / POL_CHUNK]; a
+// lane past the row end (only when nx % POL_CHUNK != 0) is reported invalid.
+__device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ chunks,
+                                 const FastDiv& dnx, uint32_t nx, uint32_t& cell, uint32_t& yz, uint32_t& x) {
+  bool ok = item < n_items;
+  uint32_t c = 0;
+  if (ok) c = chunks ? chunks[item / POL_CHUNK] + (item % POL_CHUNK) : item;
+  yz = fdiv(c, dnx);
+  x = c - yz * nx;
+  if (chunks && x < (item % POL_CHUNK)) ok = false;  // wrapped into the next row
+  cell = ok ? c : 0u;
+  if (!ok) {
+    yz = 0;
+    x = 0;
+  }
+  return ok;
+}
+
 // ---------------------------------------------------------------------------
 // wave64 reductions (DPP/permute via shfl_xor) in FP64
 // ---------------------------------------------------------------------------

@@ -26,12 +26,8 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   using R = typename C::real;
   const uint32_t lblk = logical_block(a.per_xcd);
   if (lblk >= a.nblk) return;  // whole padding block exits (uniform)
-  const uint32_t idx = lblk * BLOCK + threadIdx.x;
-  const bool inside = idx < a.N;
-  const uint32_t cell = inside ? idx : 0u;
-
-  const uint32_t yz = fdiv(cell, a.dnx);
-  const uint32_t x = cell - yz * a.nx;
+  uint32_t cell, yz, x;
+  const bool inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.dnx, a.nx, cell, yz, x);
   const uint32_t z = fdiv(yz, a.dny);
   const uint32_t y = yz - z * a.ny;
 

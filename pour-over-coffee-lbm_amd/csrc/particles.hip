@@ -24,18 +24,19 @@ namespace pol {
 // i = (int)max(0, min(n - 2, p)) with min/max in the position's type (the
 // reference's ti.min/ti.max promote the int bound to f32) and truncation,
 // frac = clamp(p - i, 0, 1).  fmin/fmax return the non-NaN operand, so a NaN
-// coordinate maps to n-2.
+// coordinate maps to n-2 with fraction 1; the trailing + 0 turns a -0 fraction
+// (p = -0) into +0 whatever sign max(0, -0) returns.
 __device__ __forceinline__ void cell_index(float p, int n, int& i, float& fr) {
   const float c = fmaxf(0.0f, fminf((float)(n - 2), p));
   i = (int)c;
   const float t = p - (float)i;
-  fr = fmaxf(0.0f, fminf(1.0f, t));
+  fr = fmaxf(0.0f, fminf(1.0f, t)) + 0.0f;
 }
 __device__ __forceinline__ void cell_index(double p, int n, int& i, double& fr) {
   const double c = fmax(0.0, fmin((double)(n - 2), p));
   i = (int)c;
   const double t = p - (double)i;
-  fr = fmax(0.0, fmin(1.0, t));
+  fr = fmax(0.0, fmin(1.0, t)) + 0.0;
 }
 
 // Schiller-Naumann with the reference's piecewise bounds:

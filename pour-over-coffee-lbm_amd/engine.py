@@ -14,6 +14,7 @@ import torch
 
 from . import _lib as L
 
+POL_CHUNK = 32  # cells per launch chunk (include/pourover_hip.h)
 _STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 
 
@@ -116,6 +117,7 @@ class HipContext:
         self.rho = torch.ones(shape, dtype=rd, device=self.device)
         self.u = torch.zeros((3,) + shape, dtype=rd, device=self.device)
         self.flags = _as_int32_view(flags).to(self.device) if flags is not None else None
+        self.chunks = self._chunk_list(flags)
         self.force = torch.zeros((3,) + shape, dtype=rd, device=self.device) if cfg.force_field else None
         self.perm = torch.full(shape, cfg.darcy_K, dtype=rd, device=self.device) if cfg.perm_field else None
         self._ctx = C.c_void_p()
@@ -134,7 +136,31 @@ class HipContext:
         b.perm = self.perm.data_ptr() if self.perm is not None else None
         b.rho = self.rho.data_ptr()
         b.u = self.u.data_ptr()
+        if self.chunks is not None:
+            b.active_chunks = self.chunks.data_ptr()
+            b.n_chunks = self.chunks.numel()
         L.check(self.lib.pol_bind(self._ctx, C.byref(b)), self._ctx, "pol_bind")
+
+    def _chunk_list(self, flags: Optional[np.ndarray]) -> Optional[torch.Tensor]:
+        """Launch list: first cells of the POL_CHUNK-cell, line-aligned row chunks
+        holding any non-solid cell (None when every chunk does or there are no
+        flags).  Solid cells are never read or written, so chunks of solid cells
+        get no threads."""
+        if flags is None:
+            return None
+        cfg = self.config
+        kind = (np.asarray(flags, dtype=np.uint32) >> L.POL_KIND_SHIFT) & L.POL_KIND_MASK
+        open_cell = kind != L.POL_KIND_SOLID
+        nch = -(-cfg.nx // POL_CHUNK)
+        pad = nch * POL_CHUNK - cfg.nx
+        if pad:
+            open_cell = np.concatenate([open_cell, np.zeros(open_cell.shape[:2] + (pad,), bool)], axis=2)
+        live = open_cell.reshape(cfg.nz, cfg.ny, nch, POL_CHUNK).any(axis=3)
+        zz, yy, cc = np.nonzero(live)
+        if zz.size == 0 or zz.size == live.size:
+            return None
+        base = ((zz.astype(np.int64) * cfg.ny + yy) * cfg.nx + cc * POL_CHUNK).astype(np.uint32)
+        return _as_int32_view(base).to(self.device)
 
     def close(self) -> None:
         if self._ctx:
@@ -173,6 +199,10 @@ class HipContext:
         if not self._use_flags:
             raise ValueError("context was created without cell flags")
         self.flags.copy_(_as_int32_view(flags).to(self.device))
+        self.chunks = self._chunk_list(flags)
+        cur = self.current
+        self._bind()
+        self.set_current(cur)
 
     def synchronize(self) -> None:
         torch.cuda.synchronize(self.device)
