@@ -64,6 +64,7 @@ enum { POL_F32 = 0, POL_F16 = 1, POL_BF16 = 2, POL_F64 = 3 };
 #define POL_FORCE_FIELD 0x20u   /* add per-cell force density buffer */
 #define POL_PERM_FIELD 0x40u    /* per-cell permeability buffer (else darcy_K) */
 #define POL_GRAVITY 0x80u       /* constant body-force density gravity[3] */
+#define POL_PARTICLE_FIELD 0x100u /* implicit particle-drag field (pol_buffers.particle_drag) */
 
 /* cell flag word (uint32 per cell) */
 #define POL_LINK_MASK 0x7FFFFu  /* bit q: source x - e_q of link q is a wall -> halfway bounce-back */
@@ -128,6 +129,12 @@ typedef struct pol_buffers {
                                     chunks (solid lanes idle) */
   uint32_t n_chunks;
   uint32_t reserved;
+  void *particle_drag; /* real [4][cells] (POL_PARTICLE_FIELD): per cell
+                          sum_p w beta_p v_p (x, y, z) and B = sum_p w beta_p
+                          over the particles whose trilinear footprint covers
+                          the cell (w = trilinear weight, beta = |drag|/|u_rel|).
+                          The collision treats the drag B (V - u) implicitly in
+                          u, like the Darcy term, so any B is stable. */
 } pol_buffers;
 #define POL_CHUNK 32 /* one 128-byte line of FP32 populations */
 
@@ -153,6 +160,12 @@ const char *pol_last_error(const pol_ctx *ctx);
 /* Attach caller-owned device buffers.  Replaces the memory-adapter attribute
  * contract f/f_new/rho/u/solid (src/core/backends/cpu_backend.py:207-212). */
 int pol_bind(pol_ctx *ctx, const pol_buffers *bufs);
+/* Replace the physics scalars (tau, cs_smag, nu_sgs_max, gravity, darcy_K,
+ * forch_CF, inlet/outlet values) of a context; nx/ny/nz, storage, options and
+ * device must equal the context's.  Takes effect for launches enqueued
+ * afterwards.  Replaces setting config module constants between runs
+ * (config/core.py:57-93, config/physics.py:255-273). */
+int pol_set_params(pol_ctx *ctx, const pol_desc *desc);
 /* Index (0/1) of the buffer holding the current state. */
 int pol_current(const pol_ctx *ctx, int *index);
 /* Select which buffer holds the current state (after a host-side restore). */
@@ -215,18 +228,19 @@ typedef struct pol_particle_buffers {
   void *drag_new, *drag_old, *drag; /* [3][count] */
   void *u_fluid;  /* [3][count] interpolated fluid velocity (diagnostic) */
   void *re_p, *cd; /* [count] diagnostics */
-  void *force;    /* [3][cells] reaction-force field (= pol_buffers.force) */
+  void *drag_field; /* [4][cells] particle-drag field (= pol_buffers.particle_drag) */
   const void *u_field; /* [3][cells] fluid velocity field read by the gather */
 } pol_particle_buffers;
 
 /* Attach a particle system.  Replaces CoffeeParticleSystem field set-up
  * (coffee_particles.py:30-64). */
 int pol_particles_bind(pol_ctx *ctx, const pol_particle_desc *d, const pol_particle_buffers *b);
-/* Interpolate, drag, under-relax and spread the reaction force.  Replaces
+/* Interpolate, drag, under-relax and spread the drag coefficient
+ * (w beta v_p, w beta) into the particle-drag field.  Replaces
  * compute_two_way_coupling_forces + apply_under_relaxation
  * (coffee_particles.py:1107-1154,1200-1212). */
 int pol_particles_couple(pol_ctx *ctx, void *stream);
-/* Clear this step's reaction-force footprint and advance the particles.
+/* Clear this step's particle-drag footprint and advance the particles.
  * Replaces clear_reaction_forces + update_particle_physics
  * (coffee_particles.py:1101-1105,641-720). */
 int pol_particles_advance(pol_ctx *ctx, void *stream);

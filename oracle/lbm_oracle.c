@@ -70,8 +70,11 @@ typedef struct {
   real rho, ux, uy, uz;
 } moments;
 
-/* Collide one gathered cell in place (post-collision written back into g). */
-static moments collide(real *g, unsigned kind, int porous, int les, real K, const real F0[3], const pol_desc *d) {
+/* Collide one gathered cell in place (post-collision written back into g).
+ * pd = particle-drag field values {sum w beta v (3), B = sum w beta} or NULL:
+ * the particles exert BV - B u with u implicit. */
+static moments collide(real *g, unsigned kind, int porous, int les, real K, const real F0[3], const real *pd,
+                       const pol_desc *d) {
   moments m;
   const real tau0 = (real)d->tau, nu0 = (real)((d->tau - 0.5) / 3.0);
   if (kind == POL_KIND_INLET) {
@@ -92,19 +95,29 @@ static moments collide(real *g, unsigned kind, int porous, int les, real K, cons
   real inv = rho > (real)1e-12 ? (real)1 / rho : (real)0;
   real F[3] = {F0[0], F0[1], F0[2]};
   real u[3];
-  real v[3] = {(jx + (real)0.5 * F[0]) * inv, (jy + (real)0.5 * F[1]) * inv, (jz + (real)0.5 * F[2]) * inv};
-  if (porous) {
-    /* rho u = j + (F0 - rho (nu/K) u - rho (C_F/sqrt K)|u| u)/2, solved for |u| */
-    real c0 = (real)1 + nu0 / ((real)2 * K);
-    real c1 = (real)d->forch_CF / ((real)2 * sqrt(K));
+  const real B = pd ? pd[3] : (real)0;
+  if (porous || B > 0) {
+    /* rho u = j + (F0 + BV - B u - rho (nu/K) u - rho (C_F/sqrt K)|u| u)/2:
+     * u (c0 + c1 |u|) = v, v = (j + F0/2 + BV/2)/rho, c0 = 1 + B/(2 rho) [+ nu/(2K)],
+     * c1 = C_F/(2 sqrt K) [porous]; |u| = 2|v| / (c0 + sqrt(c0^2 + 4 c1 |v|)) */
+    real v[3] = {(jx + (real)0.5 * (F[0] + (pd ? pd[0] : 0))) * inv,
+                 (jy + (real)0.5 * (F[1] + (pd ? pd[1] : 0))) * inv,
+                 (jz + (real)0.5 * (F[2] + (pd ? pd[2] : 0))) * inv};
+    real c0 = (real)1 + (real)0.5 * B * inv, c1 = 0, dk = 0, fk = 0;
+    if (porous) {
+      dk = nu0 / K;
+      fk = (real)d->forch_CF / sqrt(K);
+      c0 += (real)0.5 * dk;
+      c1 = (real)0.5 * fk;
+    }
     real vm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    /* |u| = 2|v| / (c0 + sqrt(c0^2 + 4 c1 |v|)), u parallel to v */
     real s = (real)2 / (c0 + sqrt(c0 * c0 + (real)4 * c1 * vm));
     for (int i = 0; i < 3; ++i) u[i] = v[i] * s;
     real un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    real c = rho * (nu0 / K + (real)d->forch_CF / sqrt(K) * un);
-    for (int i = 0; i < 3; ++i) F[i] -= c * u[i];
+    real c = rho * (dk + fk * un) + B;
+    for (int i = 0; i < 3; ++i) F[i] += (pd ? pd[i] : 0) - c * u[i];
   } else {
+    real v[3] = {(jx + (real)0.5 * F[0]) * inv, (jy + (real)0.5 * F[1]) * inv, (jz + (real)0.5 * F[2]) * inv};
     for (int i = 0; i < 3; ++i) u[i] = v[i];
   }
   m.rho = rho;
@@ -170,9 +183,10 @@ static void gather(const pol_desc *d, const uint32_t *flags, const real *fin, in
 
 /* One step: fin -> fout.  rho_out/u_out (may be NULL) receive the moments of
  * the collided state (solid cells: rho 1, u 0).  red_out (may be NULL)
- * receives the POL_NRED reductions (sums in FP64 in cell order). */
-int SFX(oracle_step)(const pol_desc *d, const uint32_t *flags, const real *force, const real *perm, const real *fin,
-                     real *fout, real *rho_out, real *u_out, double *red_out) {
+ * receives the POL_NRED reductions (sums in FP64 in cell order).  pdrag
+ * (may be NULL) is the [4][cells] particle-drag field. */
+int SFX(oracle_step)(const pol_desc *d, const uint32_t *flags, const real *force, const real *perm, const real *pdrag,
+                     const real *fin, real *fout, real *rho_out, real *u_out, double *red_out) {
   const size_t N = (size_t)d->nx * d->ny * d->nz;
   const int use_flags = (d->options & POL_USE_FLAGS) && flags;
   double red[POL_NRED];
@@ -209,7 +223,11 @@ int SFX(oracle_step)(const pol_desc *d, const uint32_t *flags, const real *force
         real K = (real)d->darcy_K;
         if (porous && (d->options & POL_PERM_FIELD) && perm) K = perm[cell];
         const int les = (d->options & POL_LES) && !(fl & POL_FLAG_NO_LES);
-        moments m = collide(g, kind, porous, les, K, F0, d);
+        real pd[4];
+        const int has_pd = (d->options & POL_PARTICLE_FIELD) && pdrag;
+        if (has_pd)
+          for (int i = 0; i < 4; ++i) pd[i] = pdrag[(size_t)i * N + cell];
+        moments m = collide(g, kind, porous, les, K, F0, has_pd ? pd : NULL, d);
         if (fout)
           for (int q = 0; q < 19; ++q) fout[(size_t)q * N + cell] = g[q];
         if (rho_out) rho_out[cell] = m.rho;
@@ -366,7 +384,10 @@ typedef struct {
   real rho_fluid, nu_fluid, drag_max_per_mass, alpha, g[3], dt, max_disp, wall_damping;
 } oracle_particle_params;
 
-/* couple: gather u, drag, scatter -drag into force (accumulated), relax. */
+/* couple: gather u, drag = beta (u_f - v), accumulate the [4][cells] drag field
+ * (w beta v, w beta) in `force`, relax.  The reference scatters -drag
+ * explicitly (coffee_particles.py:1148-1149); the field form lets the
+ * collision treat the reaction implicitly. */
 void SFX(oracle_particles_couple)(int nx, int ny, int nz, const oracle_particle_params *pp, const real *pos,
                                   const real *vel, const real *radius, const real *mass, const int32_t *active,
                                   const real *u_field, real *force, real *u_fluid, real *drag_new, real *drag_old,
@@ -419,9 +440,12 @@ void SFX(oracle_particles_couple)(int nx, int ny, int nz, const oracle_particle_
       if (pp->drag_law == POL_DRAG_TEXTBOOK) dm *= mag;
       real cap = mass[p] * pp->drag_max_per_mass;
       if (dm > cap) dm = cap;
-      for (int d = 0; d < 3; ++d) dn[d] = dm * rel[d] / mag;
-      for (int d = 0; d < 3; ++d)
-        for (int k = 0; k < 8; ++k) force[(size_t)d * N + cs[k]] += ws[k] * (-dn[d]);
+      const real beta = dm / mag;
+      for (int d = 0; d < 3; ++d) dn[d] = beta * rel[d];
+      /* drag field: w beta v (3 components) and w beta */
+      const real comp[4] = {beta * vel[p], beta * vel[P + p], beta * vel[2 * P + p], beta};
+      for (int d = 0; d < 4; ++d)
+        for (int k = 0; k < 8; ++k) force[(size_t)d * N + cs[k]] += ws[k] * comp[d];
     } else {
       re_out[p] = 0;
       cd_out[p] = 0;
@@ -457,7 +481,7 @@ void SFX(oracle_particles_advance)(int nx, int ny, int nz, const oracle_particle
     const size_t c000 = (size_t)ii[2] * nxy + (size_t)ii[1] * nx + ii[0];
     const size_t cs[8] = {c000, c000 + 1, c000 + nx, c000 + nx + 1, c000 + nxy, c000 + nxy + 1, c000 + nxy + nx,
                           c000 + nxy + nx + 1};
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 4; ++d)
       for (int k = 0; k < 8; ++k) force[(size_t)d * N + cs[k]] = 0;
     real v[3], uf[3], dn[3], x[3];
     for (int d = 0; d < 3; ++d) {

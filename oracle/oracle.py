@@ -20,6 +20,7 @@ LIB = HERE / "liboracle_lbm.so"
 # oracle must not import the product package)
 POL_PERIODIC_X, POL_PERIODIC_Y, POL_PERIODIC_Z = 0x1, 0x2, 0x4
 POL_USE_FLAGS, POL_LES, POL_FORCE_FIELD, POL_PERM_FIELD, POL_GRAVITY = 0x8, 0x10, 0x20, 0x40, 0x80
+POL_PARTICLE_FIELD = 0x100
 NRED = 12
 
 
@@ -103,8 +104,9 @@ def _sfx(prec):
     return "_f64" if prec == "f64" else "_f32"
 
 
-def step(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, want_macro=False, prec="f64"):
-    """One oracle step.  Returns (f_new, rho, u, red) (rho/u None unless want_macro)."""
+def step(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, want_macro=False, prec="f64", pdrag=None):
+    """One oracle step.  Returns (f_new, rho, u, red) (rho/u None unless want_macro).
+    pdrag: [4, nz, ny, nx] particle-drag field (used when desc has POL_PARTICLE_FIELD)."""
     dt = _dtype(prec)
     f = np.ascontiguousarray(f, dtype=dt)
     fout = np.zeros_like(f)
@@ -115,9 +117,10 @@ def step(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, want_macr
     fl = np.ascontiguousarray(flags, dtype=np.uint32) if flags is not None else None
     fo = np.ascontiguousarray(force, dtype=dt) if force is not None else None
     pm = np.ascontiguousarray(perm, dtype=dt) if perm is not None else None
+    pd = np.ascontiguousarray(pdrag, dtype=dt) if pdrag is not None else None
     fn = getattr(lib(), "oracle_step" + _sfx(prec))
     fn.restype = C.c_int
-    rc = fn(C.byref(desc), _p(fl), _p(fo), _p(pm), _p(f), _p(fout), _p(rho), _p(u), _p(red))
+    rc = fn(C.byref(desc), _p(fl), _p(fo), _p(pm), _p(pd), _p(f), _p(fout), _p(rho), _p(u), _p(red))
     if rc != 0:
         raise RuntimeError("oracle_step failed")
     return fout, rho, u, red
@@ -131,9 +134,9 @@ def run(desc: Desc, f: np.ndarray, nsteps: int, flags=None, force=None, perm=Non
     return f, rho, u, red
 
 
-def moments(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, prec="f64"):
+def moments(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, prec="f64", pdrag=None):
     """Moments (rho, u, red) of the state f (the gather of the next step)."""
-    _, rho, u, red = step(desc, f, flags, force, perm, want_macro=True, prec=prec)
+    _, rho, u, red = step(desc, f, flags, force, perm, want_macro=True, prec=prec, pdrag=pdrag)
     return rho, u, red
 
 

@@ -29,6 +29,7 @@ POL_LES = 0x10
 POL_FORCE_FIELD = 0x20
 POL_PERM_FIELD = 0x40
 POL_GRAVITY = 0x80
+POL_PARTICLE_FIELD = 0x100
 POL_LINK_MASK = 0x7FFFF
 POL_KIND_SHIFT = 24
 POL_KIND_MASK = 0x7
@@ -61,6 +62,7 @@ class PolBuffers(C.Structure):
         ("f", C.c_void_p * 2), ("cell_flags", C.c_void_p), ("force", C.c_void_p),
         ("perm", C.c_void_p), ("rho", C.c_void_p), ("u", C.c_void_p),
         ("active_chunks", C.c_void_p), ("n_chunks", C.c_uint32), ("reserved", C.c_uint32),
+        ("particle_drag", C.c_void_p),
     ]
 
 
@@ -76,7 +78,7 @@ class PolParticleDesc(C.Structure):
 class PolParticleBuffers(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "pos", "vel", "radius", "mass", "active", "drag_new", "drag_old", "drag", "u_fluid", "re_p", "cd",
-        "force", "u_field")]
+        "drag_field", "u_field")]
 
 
 class PolAdjointBuffers(C.Structure):
@@ -93,6 +95,7 @@ SIGNATURES = {
     "pol_destroy": (C.c_int, [_P]),
     "pol_last_error": (C.c_char_p, [_P]),
     "pol_bind": (C.c_int, [_P, C.POINTER(PolBuffers)]),
+    "pol_set_params": (C.c_int, [_P, C.POINTER(PolDesc)]),
     "pol_current": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "pol_set_current": (C.c_int, [_P, C.c_int]),
     "pol_init_equilibrium": (C.c_int, [_P, _P, _P, _P]),

@@ -1,0 +1,151 @@
+"""Differentiable V60 run: outlet flux gradient by a hand-written adjoint.
+
+Objective (lattice units): J = sum over the last `window` steps of the
+outlet mass flux, sum over outlet cells of -rho u_z (the moments each step
+collides; the reduction slot POL_RED_OUTFLUX).  Parameters: the per-cell
+permeability K of the filter layer and the inlet velocity u_in.
+
+The reverse sweep runs csrc/adjoint.hip one step at a time, newest first.
+Forward states are rebuilt from checkpoints every `checkpoint` steps: a
+segment is re-run forward from its checkpoint into a segment store (one
+fused step per state, pol_step_from) and then swept in reverse, so memory
+holds the checkpoints plus one segment (128^3, 50-step segments: 8 GB in
+FP32), far inside 288 GB of HBM.
+
+There is no adjoint in the reference (no autograd or checkpointing exists in
+src/); OutletFluxFunction wraps the sweep as a torch.autograd.Function so
+callers can treat the whole run as a differentiable op.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .engine import HipContext
+from .workloads import V60Workload
+
+
+class AdjointProblem:
+    def __init__(self, wl: V60Workload, K: np.ndarray, steps: int = 500, window: int = 100, checkpoint: int = 50):
+        cfg = wl.config
+        if not cfg.perm_field:
+            raise ValueError("the adjoint workload needs SolverConfig(perm_field=True)")
+        if cfg.precision not in ("f32", "f64"):
+            raise ValueError("the adjoint runs with f32 or f64 populations")
+        if steps % checkpoint:
+            raise ValueError("steps must be a multiple of the checkpoint interval")
+        self.wl = wl
+        self.steps, self.window, self.checkpoint = steps, window, checkpoint
+        self.ctx = HipContext(cfg, wl.geometry.flags)
+        self.ctx.perm.copy_(torch.as_tensor(K, dtype=cfg.real_dtype))
+        shape = self.ctx.shape
+        rd = cfg.real_dtype
+        dev = self.ctx.device
+        self.lam = [torch.zeros((19,) + shape, dtype=rd, device=dev) for _ in range(2)]
+        self.grad_perm = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.rev_flags = torch.from_numpy(np.ascontiguousarray(wl.geometry.rev_flags).view(np.int32)).to(dev)
+        b = L.PolAdjointBuffers()
+        b.lam[0] = self.lam[0].data_ptr()
+        b.lam[1] = self.lam[1].data_ptr()
+        b.grad_perm = self.grad_perm.data_ptr()
+        b.rev_flags = self.rev_flags.data_ptr()
+        L.check(self.ctx.lib.pol_adjoint_bind(self.ctx.handle, C.byref(b)), self.ctx.handle, "pol_adjoint_bind")
+        nseg = steps // checkpoint
+        sd = cfg.storage_dtype
+        self.ckpt = torch.empty((nseg, 19) + shape, dtype=sd, device=dev)
+        self.seg = torch.empty((checkpoint, 19) + shape, dtype=sd, device=dev)
+        self.timings = {}
+
+    # ---- parameters -------------------------------------------------------------
+    def set_permeability(self, K) -> None:
+        self.ctx.perm.copy_(torch.as_tensor(K, dtype=self.ctx.config.real_dtype))
+
+    def set_inlet_velocity(self, u: Sequence[float]) -> None:
+        self.ctx.set_params(inlet_u=tuple(float(v) for v in u))
+
+    # ---- forward ------------------------------------------------------------------
+    def objective_step(self, n: int) -> bool:
+        return n >= self.steps - self.window
+
+    def forward(self, checkpoints: bool = False) -> float:
+        """J (FP64) of one full run from rest; optionally store the checkpoints."""
+        ctx = self.ctx
+        ctx.set_current(0)
+        ctx.init_equilibrium()
+        J = 0.0
+        for n in range(self.steps):
+            if checkpoints and n % self.checkpoint == 0:
+                self.ckpt[n // self.checkpoint].copy_(ctx.f_current)
+            if self.objective_step(n):
+                red = ctx.step(1, L.POL_WANT_REDUCE)
+                J += float(red[7])
+            else:
+                ctx.step(1)
+        return J
+
+    def forward_time(self) -> float:
+        """Seconds for the plain forward run (no reductions, no checkpoints)."""
+        ctx = self.ctx
+        ctx.set_current(0)
+        ctx.init_equilibrium()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.step(self.steps)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # ---- reverse ------------------------------------------------------------------
+    def gradient(self):
+        """(J, dJ/dK [nz,ny,nx] float64 tensor, dJ/du_in [3]) by checkpointed adjoint."""
+        ctx, lib = self.ctx, self.ctx.lib
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = self.forward(checkpoints=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.check(lib.pol_adjoint_reset(ctx.handle, L.stream_handle()), ctx.handle, "pol_adjoint_reset")
+        K = self.checkpoint
+        for s in reversed(range(self.steps // K)):
+            # rebuild f^{sK} .. f^{sK+K-1} into the segment store
+            self.seg[0].copy_(self.ckpt[s])
+            for i in range(1, K):
+                ctx.step_from(self.seg[i - 1], self.seg[i])
+            for i in reversed(range(K)):
+                n = s * K + i
+                st = lib.pol_adjoint_step(ctx.handle, L.ptr(self.seg[i]), int(self.objective_step(n)),
+                                          L.stream_handle())
+                L.check(st, ctx.handle, "pol_adjoint_step")
+        out = np.zeros(4, dtype=np.float64)
+        L.check(lib.pol_adjoint_collect(ctx.handle, out.ctypes.data_as(C.c_void_p), L.stream_handle()), ctx.handle,
+                "pol_adjoint_collect")
+        t2 = time.perf_counter()
+        self.timings = {"forward_s": t1 - t0, "reverse_s": t2 - t1, "J_adjoint": float(out[3])}
+        return J, self.grad_perm.clone(), out[:3].copy()
+
+
+class OutletFluxFunction(torch.autograd.Function):
+    """J(K, u_in) as an autograd op; backward runs the checkpointed adjoint."""
+
+    @staticmethod
+    def forward(ctx, K: torch.Tensor, u_in: torch.Tensor, problem: AdjointProblem):
+        problem.set_permeability(K.detach())
+        problem.set_inlet_velocity(u_in.detach().double().cpu().tolist())
+        J, gK, gu = problem.gradient()
+        ctx.save_for_backward(gK.to(device=K.device, dtype=K.dtype),
+                              torch.as_tensor(gu, dtype=u_in.dtype, device=u_in.device))
+        return torch.tensor(J, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        gK, gu = ctx.saved_tensors
+        g = grad_out.to(gK.device)
+        return g.to(gK.dtype) * gK, g.to(gu.dtype) * gu, None
+
+
+def outlet_flux(K: torch.Tensor, u_in: torch.Tensor, problem: AdjointProblem) -> torch.Tensor:
+    return OutletFluxFunction.apply(K, u_in, problem)

@@ -135,6 +135,8 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
   if ((o & POL_USE_FLAGS) && !b->cell_flags) return fail(c, POL_ERR_INVALID, "pol_bind: POL_USE_FLAGS without cell_flags");
   if ((o & POL_FORCE_FIELD) && !b->force) return fail(c, POL_ERR_INVALID, "pol_bind: POL_FORCE_FIELD without force buffer");
   if ((o & POL_PERM_FIELD) && !b->perm) return fail(c, POL_ERR_INVALID, "pol_bind: POL_PERM_FIELD without perm buffer");
+  if ((o & POL_PARTICLE_FIELD) && !b->particle_drag)
+    return fail(c, POL_ERR_INVALID, "pol_bind: POL_PARTICLE_FIELD without particle_drag buffer");
   if ((o & POL_PERM_FIELD) && !(o & POL_USE_FLAGS)) return fail(c, POL_ERR_INVALID, "pol_bind: permeability needs cell flags");
   if (b->active_chunks && (b->n_chunks == 0 || (uint64_t)b->n_chunks * POL_CHUNK > (uint64_t)c->N + POL_CHUNK))
     return fail(c, POL_ERR_INVALID, "pol_bind: n_chunks out of range");
@@ -146,6 +148,16 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
   c->bound = true;
   c->cur = 0;
   c->err.clear();
+  return POL_OK;
+}
+
+int pol_set_params(pol_ctx* c, const pol_desc* d) {
+  if (!c || !d) return POL_ERR_INVALID;
+  if (d->nx != c->d.nx || d->ny != c->d.ny || d->nz != c->d.nz || d->storage != c->d.storage ||
+      d->options != c->d.options || d->device != c->d.device)
+    return fail(c, POL_ERR_INVALID, "pol_set_params: geometry, storage, options and device are fixed per context");
+  if (!(d->tau > 0.5) || !std::isfinite(d->tau)) return fail(c, POL_ERR_INVALID, "pol_set_params: tau must be > 1/2");
+  c->d = *d;
   return POL_OK;
 }
 
@@ -177,6 +189,7 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.flags = (d.options & POL_USE_FLAGS) ? c->b.cell_flags : nullptr;
   a.force = c->b.force;
   a.perm = c->b.perm;
+  a.pdrag = c->b.particle_drag;
   a.rho = c->b.rho;
   a.u = c->b.u;
   a.partial = c->partial;
@@ -278,10 +291,10 @@ int pol_reduce(pol_ctx* c, double* red_out, void* stream) {
 int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particle_buffers* b) {
   if (!c || !d || !b) return POL_ERR_INVALID;
   if (d->count < 0) return fail(c, POL_ERR_INVALID, "pol_particles_bind: negative count");
-  if (!c->bound || !(c->d.options & POL_FORCE_FIELD))
-    return fail(c, POL_ERR_STATE, "pol_particles_bind: bind forward buffers with POL_FORCE_FIELD first");
+  if (!c->bound || !(c->d.options & POL_PARTICLE_FIELD))
+    return fail(c, POL_ERR_STATE, "pol_particles_bind: bind forward buffers with POL_PARTICLE_FIELD first");
   if (d->count > 0 && (!b->pos || !b->vel || !b->radius || !b->mass || !b->active || !b->drag_new || !b->drag_old ||
-                       !b->drag || !b->u_fluid || !b->re_p || !b->cd || !b->force || !b->u_field))
+                       !b->drag || !b->u_fluid || !b->re_p || !b->cd || !b->drag_field || !b->u_field))
     return fail(c, POL_ERR_INVALID, "pol_particles_bind: missing particle buffer");
   if (!(d->relax_alpha >= 0.0 && d->relax_alpha <= 1.0)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: alpha outside [0,1]");
   if (!(d->dt > 0.0) || !(d->max_disp > 0.0)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: dt and max_disp must be > 0");
@@ -309,7 +322,7 @@ int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particl
   p.u_fluid = b->u_fluid;
   p.re_p = b->re_p;
   p.cd = b->cd;
-  p.force = b->force;
+  p.drag_field = b->drag_field;
   p.u_field = b->u_field;
   p.rho_fluid = d->rho_fluid;
   p.nu_fluid = d->nu_fluid;
@@ -352,6 +365,8 @@ int pol_adjoint_bind(pol_ctx* c, const pol_adjoint_buffers* b) {
   if (!c || !b) return POL_ERR_INVALID;
   if (c->d.storage != POL_F32 && c->d.storage != POL_F64)
     return fail(c, POL_ERR_UNSUPPORTED, "pol_adjoint_bind: adjoint supports F32 and F64 storage");
+  if (c->d.options & POL_PARTICLE_FIELD)
+    return fail(c, POL_ERR_UNSUPPORTED, "pol_adjoint_bind: the adjoint does not cover the particle-drag field");
   if (!(c->d.options & POL_USE_FLAGS) || !c->bound)
     return fail(c, POL_ERR_STATE, "pol_adjoint_bind: bind forward buffers with cell flags first");
   if (!b->lam[0] || !b->lam[1] || b->lam[0] == b->lam[1] || !b->rev_flags)

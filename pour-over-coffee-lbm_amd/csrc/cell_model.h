@@ -117,7 +117,14 @@ __device__ inline void set_equilibrium(R (&d)[Q], R rd, R rho, R ux, R uy, R uz)
 
 // Collide one gathered cell in place: d[] in = pre-collision scaled
 // deviations D = 36 (f - w), out = post-collision ones.  kind/porous from the
-// cell flags; K = permeability, (fx,fy,fz) = extra force density.
+// cell flags; K = permeability, (fx,fy,fz) = extra force density; (bx,by,bz,B)
+// = particle-drag field: the particles exert F_p = BV - B u on the cell.
+//
+// Drag-type forces are implicit in u: with Guo's rho u = m + F/2 and
+// F = F0 + (BV - B u) - rho (nu/K + C_F |u| / sqrt K) u (porous cells only),
+//   u (c0 + c1 |u|) = v,  v = (m + F0/2 + BV/2) / rho,
+//   c0 = 1 + B / (2 rho) + nu / (2K),  c1 = C_F / (2 sqrt K),
+// solved in closed form (Guo & Zhao 2002) -- stable for any K and any B.
 //
 // Evaluated pairwise over opposite directions: equilibrium and Guo source of
 // a pair share their symmetric part and negate the antisymmetric one, and the
@@ -125,7 +132,7 @@ __device__ inline void set_equilibrium(R (&d)[Q], R rd, R rho, R ux, R uy, R uz)
 // halves the arithmetic of a per-direction evaluation.
 template <typename R, bool FULL>
 __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous, bool les, R K, R fx, R fy, R fz,
-                                          const Phys<R>& P) {
+                                          R bx, R by, R bz, R B, const Phys<R>& P) {
   CellOut<R> o;
   if (FULL && kind == POL_KIND_INLET) {
     o.rd = P.inlet_rho - R(1);
@@ -160,15 +167,28 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
     Fz = P.G[2] + fz;
   }
   R ux, uy, uz;
-  if (FULL && porous) {
-    R dk, fk;
-    porous_velocity<R>((mx + R(18) * Fx) * inv36, (my + R(18) * Fy) * inv36, (mz + R(18) * Fz) * inv36, P.nu0, K,
-                       P.CF, ux, uy, uz, dk, fk);
+  if (FULL && (porous || B > R(0))) {
+    const R vx = (mx + R(18) * (Fx + bx)) * inv36;
+    const R vy = (my + R(18) * (Fy + by)) * inv36;
+    const R vz = (mz + R(18) * (Fz + bz)) * inv36;
+    R c0 = R(1) + R(18) * B * inv36, c1 = R(0), dk = R(0), fk = R(0);
+    if (porous) {
+      const R sqK = sqrt(K);
+      dk = P.nu0 / K;    // Darcy coefficient
+      fk = P.CF / sqK;   // Forchheimer coefficient
+      c0 += R(0.5) * dk;
+      c1 = R(0.5) * fk;
+    }
+    const R vmag = sqrt(vx * vx + vy * vy + vz * vz);
+    const R s = R(2) / (c0 + sqrt(c0 * c0 + R(4) * c1 * vmag));
+    ux = vx * s;
+    uy = vy * s;
+    uz = vz * s;
     const R umag = sqrt(ux * ux + uy * uy + uz * uz);
-    const R c = rho * (dk + fk * umag);
-    Fx -= c * ux;
-    Fy -= c * uy;
-    Fz -= c * uz;
+    const R c = rho * (dk + fk * umag) + B;
+    Fx += bx - c * ux;
+    Fy += by - c * uy;
+    Fz += bz - c * uz;
   } else {
     ux = (mx + R(18) * Fx) * inv36;
     uy = (my + R(18) * Fy) * inv36;
@@ -216,18 +236,21 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
   //   S_q = (1 - w/2) c_q [3 (e_q - u).F + 9 (e_q.u)(e_q.F)]
   const R cf = FULL ? (R(1) - R(0.5) * omega) : R(0);
   const R uF = FULL ? (ux * Fx + uy * Fy + uz * Fz) : R(0);
-  d[0] = keep * d[0] + omega * d0eq - (FULL ? R(3) * cf * CW<R>(0) * uF : R(0));
+  d[0] = keep * d[0] + omega * d0eq - (FULL ? cf * (R(3) * CW<R>(0) * uF) : R(0));
   static_for<0, 9>([&](auto pc) {
     constexpr int a = PAIR_A(decltype(pc)::value);
     constexpr int b = OPP(a);
     constexpr R c = CW<R>(a);
     const R eu = R(EX(a)) * ux + R(EY(a)) * uy + R(EZ(a)) * uz;
-    R sym = omega * c * (base + R(4.5) * rho * eu * eu);
-    R anti = omega * R(3) * c * rho * eu;
+    // omega multiplies the finished equilibrium terms: a rounded product such
+    // as omega*3 would scale every cell's equilibrium momentum by the same
+    // (1 + eps) and drift the flow systematically
+    R sym = omega * (c * (base + R(4.5) * rho * eu * eu));
+    R anti = omega * (R(3) * c * rho * eu);
     if constexpr (FULL) {
       const R eF = R(EX(a)) * Fx + R(EY(a)) * Fy + R(EZ(a)) * Fz;
-      sym += cf * c * (R(9) * eu * eF - R(3) * uF);
-      anti += R(3) * cf * c * eF;
+      sym += cf * (c * (R(9) * eu * eF - R(3) * uF));
+      anti += cf * (R(3) * c * eF);
     }
     d[a] = keep * d[a] + sym + anti;
     d[b] = keep * d[b] + sym - anti;

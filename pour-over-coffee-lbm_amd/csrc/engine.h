@@ -15,6 +15,7 @@ struct StepArgs {
   const uint32_t* flags;
   const void* force;  // real [3][N]
   const void* perm;   // real [N]
+  const void* pdrag;  // real [4][N] particle-drag field (sum w beta v, sum w beta)
   void* rho;          // real [N]
   void* u;            // real [3][N]
   double* partial;    // [nblk][POL_NRED]
@@ -49,7 +50,7 @@ struct ParticleArgs {
   void *pos, *vel, *radius, *mass;
   int32_t* active;
   void *drag_new, *drag_old, *drag, *u_fluid, *re_p, *cd;
-  void* force;
+  void* drag_field;  // [4][N]
   const void* u_field;
   double rho_fluid, nu_fluid, drag_max_per_mass, alpha;
   double g[3];

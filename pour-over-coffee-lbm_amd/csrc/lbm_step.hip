@@ -83,12 +83,20 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
       });
     }
     R fx = R(0), fy = R(0), fz = R(0), K = P.K;
+    R bx = R(0), by = R(0), bz = R(0), B = R(0);
     if constexpr (FULL) {
       if (a.options & POL_FORCE_FIELD) {
         const R* F = (const R*)a.force;
         fx = F[cell];
         fy = F[(size_t)N + cell];
         fz = F[2 * (size_t)N + cell];
+      }
+      if (a.options & POL_PARTICLE_FIELD) {
+        const R* Pd = (const R*)a.pdrag;
+        bx = Pd[cell];
+        by = Pd[(size_t)N + cell];
+        bz = Pd[2 * (size_t)N + cell];
+        B = Pd[3 * (size_t)N + cell];
       }
     }
     const bool porous = FULL && (fl & POL_FLAG_POROUS);
@@ -97,14 +105,14 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     }
     const bool les = FULL && (a.options & POL_LES) && !(fl & POL_FLAG_NO_LES);
     if constexpr (STORE) {
-      o = collide_cell<R, FULL>(f, kind, porous, les, K, fx, fy, fz, P);
+      o = collide_cell<R, FULL>(f, kind, porous, les, K, fx, fy, fz, bx, by, bz, B, P);
       static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         C::store(fout + (size_t)q * N + cell, f[q]);
       });
     } else {
       // moments only: the same velocity definition the collision uses
-      o = collide_cell<R, FULL>(f, kind, porous, false, K, fx, fy, fz, P);
+      o = collide_cell<R, FULL>(f, kind, porous, false, K, fx, fy, fz, bx, by, bz, B, P);
     }
   }
 
@@ -190,7 +198,8 @@ __global__ void __launch_bounds__(256) k_init_equilibrium(S* __restrict__ f, con
 template <typename S>
 static hipError_t dispatch_sc(const StepArgs& a, bool store_f, const Grid& g, hipStream_t s) {
   const bool flags = a.flags != nullptr;
-  const bool full = flags || (a.options & (POL_LES | POL_FORCE_FIELD | POL_PERM_FIELD | POL_GRAVITY));
+  const bool full =
+      flags || (a.options & (POL_LES | POL_FORCE_FIELD | POL_PERM_FIELD | POL_GRAVITY | POL_PARTICLE_FIELD));
   dim3 grid(g.launch), block(BLOCK);
   if (store_f) {
     if (flags)

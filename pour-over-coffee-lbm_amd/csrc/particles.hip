@@ -3,11 +3,16 @@
 // for the FP64 oracle run).
 //
 // couple : particle -> cell index + clamped trilinear fractions, gather of the
-//          fluid velocity at the 8 corners, Schiller-Naumann drag, drag
-//          under-relaxation, and the reaction -drag spread to the 8 corners
-//          with hardware floating-point atomics into the force-density field
-//          that the next stream-collide step reads.
-// advance: zero this step's 8-corner footprint in the force field (so the
+//          fluid velocity at the 8 corners, Schiller-Naumann drag
+//          drag = beta (u_f - v), drag under-relaxation, and the reaction
+//          spread to the 8 corners with hardware floating-point atomics as
+//          the pair (w beta v, w beta) into the particle-drag field; the next
+//          stream-collide step applies sum w beta (v - u) with the cell
+//          velocity u implicit.  (The reference scatters -drag explicitly,
+//          coffee_particles.py:1148-1149; with the lognormal size tail of the
+//          brew workload, particles up to 6 cells wide give beta ~ 3 per cell
+//          and the explicit reaction diverges within ~30 steps.)
+// advance: zero this step's 8-corner footprint in the drag field (so the
 //          field never needs a full-grid clear), then move the particle
 //          (linearised-implicit drag, buoyancy-corrected gravity, displacement
 //          clamp, rejection at walls and at the filter layer, removal at
@@ -63,7 +68,7 @@ __global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) 
   const R* pos = (const R*)a.pos;
   const R* vel = (const R*)a.vel;
   const R* ufield = (const R*)a.u_field;
-  R* force = (R*)a.force;
+  R* field = (R*)a.drag_field;
   const int P = a.count;
   const R px = pos[p], py = pos[P + p], pz = pos[2 * P + p];
   int i, j, k;
@@ -108,23 +113,26 @@ __global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) 
     R dmag = R(0.5) * rho_f * cd * area * mag;
     if (a.drag_law == POL_DRAG_TEXTBOOK) dmag *= mag;
     dmag = fmin(dmag, ((const R*)a.mass)[p] * (R)a.drag_max_per_mass);
-    const R s = dmag / mag;
-    dn[0] = s * rx;
-    dn[1] = s * ry;
-    dn[2] = s * rz;
-    // Newton's third law: -drag onto the 8 corners
+    const R beta = dmag / mag;  // drag = beta (u_f - v)
+    dn[0] = beta * rx;
+    dn[1] = beta * ry;
+    dn[2] = beta * rz;
+    // Newton's third law, fluid side implicit: the corners receive w beta v
+    // and w beta; the collision applies sum w beta (v - u_cell) with u_cell
+    // solved implicitly (cell_model.h)
+    const R comp[4] = {beta * vel[p], beta * vel[P + p], beta * vel[2 * P + p], beta};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      R* F = force + (size_t)d * N;
-      const R rf = -dn[d];
-      atomic_add(F + c000, w000 * rf);
-      atomic_add(F + c010, w010 * rf);
-      atomic_add(F + c001, w001 * rf);
-      atomic_add(F + c011, w011 * rf);
-      atomic_add(F + c100, w100 * rf);
-      atomic_add(F + c110, w110 * rf);
-      atomic_add(F + c101, w101 * rf);
-      atomic_add(F + c111, w111 * rf);
+    for (int d = 0; d < 4; ++d) {
+      R* F = field + (size_t)d * N;
+      const R v = comp[d];
+      atomic_add(F + c000, w000 * v);
+      atomic_add(F + c010, w010 * v);
+      atomic_add(F + c001, w001 * v);
+      atomic_add(F + c011, w011 * v);
+      atomic_add(F + c100, w100 * v);
+      atomic_add(F + c110, w110 * v);
+      atomic_add(F + c101, w101 * v);
+      atomic_add(F + c111, w111 * v);
     }
   } else {
     re_p[p] = R(0);
@@ -151,11 +159,11 @@ __global__ void __launch_bounds__(256) k_particles_advance(const ParticleArgs a)
   if (!active[p]) return;
   R* pos = (R*)a.pos;
   R* vel = (R*)a.vel;
-  R* force = (R*)a.force;
+  R* field = (R*)a.drag_field;
   const int P = a.count;
   const uint32_t nx = a.nx, ny = a.ny, nz = a.nz, nxy = a.nxy, N = a.N;
   const R x[3] = {pos[p], pos[P + p], pos[2 * P + p]};
-  // 1. clear the reaction footprint written by couple (same corners)
+  // 1. clear the drag-field footprint written by couple (same corners)
   {
     int i, j, k;
     R t;
@@ -166,9 +174,9 @@ __global__ void __launch_bounds__(256) k_particles_advance(const ParticleArgs a)
     const uint32_t cs[8] = {c000, c000 + 1, c000 + nx, c000 + nx + 1, c000 + nxy, c000 + nxy + 1, c000 + nxy + nx,
                             c000 + nxy + nx + 1};
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 4; ++d)
 #pragma unroll
-      for (int c = 0; c < 8; ++c) force[(size_t)d * N + cs[c]] = R(0);
+      for (int c = 0; c < 8; ++c) field[(size_t)d * N + cs[c]] = R(0);
   }
   // 2. velocity: drag linearised about the current relative velocity,
   //    dv/dt = beta/m (u_f - v) + g, implicit in beta (stable for any

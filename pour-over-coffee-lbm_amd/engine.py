@@ -39,6 +39,7 @@ class SolverConfig:
     outlet_rho: float = 1.0
     force_field: bool = False
     perm_field: bool = False
+    particle_field: bool = False     # implicit particle-drag field (two-way coupling)
     device: int = 0
 
     @property
@@ -68,6 +69,7 @@ class SolverConfig:
         o |= L.POL_FORCE_FIELD if self.force_field else 0
         o |= L.POL_PERM_FIELD if self.perm_field else 0
         o |= L.POL_GRAVITY if self.gravity is not None else 0
+        o |= L.POL_PARTICLE_FIELD if self.particle_field else 0
         return o
 
     def desc(self, use_flags: bool) -> L.PolDesc:
@@ -120,6 +122,7 @@ class HipContext:
         self.chunks = self._chunk_list(flags)
         self.force = torch.zeros((3,) + shape, dtype=rd, device=self.device) if cfg.force_field else None
         self.perm = torch.full(shape, cfg.darcy_K, dtype=rd, device=self.device) if cfg.perm_field else None
+        self.pdrag = torch.zeros((4,) + shape, dtype=rd, device=self.device) if cfg.particle_field else None
         self._ctx = C.c_void_p()
         self._use_flags = flags is not None
         st = self.lib.pol_create(C.byref(self._ctx), C.byref(cfg.desc(self._use_flags)))
@@ -139,6 +142,7 @@ class HipContext:
         if self.chunks is not None:
             b.active_chunks = self.chunks.data_ptr()
             b.n_chunks = self.chunks.numel()
+        b.particle_drag = self.pdrag.data_ptr() if self.pdrag is not None else None
         L.check(self.lib.pol_bind(self._ctx, C.byref(b)), self._ctx, "pol_bind")
 
     def _chunk_list(self, flags: Optional[np.ndarray]) -> Optional[torch.Tensor]:
@@ -206,6 +210,16 @@ class HipContext:
 
     def synchronize(self) -> None:
         torch.cuda.synchronize(self.device)
+
+    def set_params(self, **changes) -> None:
+        """Change physics scalars (tau, cs_smag, gravity, darcy_K, forch_CF, inlet_u, ...)."""
+        from dataclasses import replace
+
+        cfg = replace(self.config, **changes)
+        if cfg.options(self._use_flags) != self.config.options(self._use_flags):
+            raise ValueError("set_params cannot switch options (LES/gravity/fields/periodicity)")
+        L.check(self.lib.pol_set_params(self._ctx, C.byref(cfg.desc(self._use_flags))), self._ctx, "pol_set_params")
+        self.config = cfg
 
     # -- launches --------------------------------------------------------------
     def init_equilibrium(self, rho: Optional[torch.Tensor] = None, u: Optional[torch.Tensor] = None) -> None:

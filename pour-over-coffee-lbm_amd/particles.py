@@ -33,8 +33,8 @@ class HipCoffeeParticleSystem:
                  rho_fluid: float = 1.0, relax_alpha: float = 0.8, gravity=(0.0, 0.0, 0.0), dt: float = 1.0,
                  max_disp: float = 1.0, wall_damping: float = 0.3, drag_law: int = L.POL_DRAG_TEXTBOOK,
                  drag_max_per_mass: float = 1.0):
-        if ctx.force is None:
-            raise ValueError("the lattice needs a force field (SolverConfig.force_field=True)")
+        if ctx.pdrag is None:
+            raise ValueError("the lattice needs the particle-drag field (SolverConfig.particle_field=True)")
         self.ctx = ctx
         self.lib = ctx.lib
         rd = ctx.config.real_dtype
@@ -74,7 +74,7 @@ class HipCoffeeParticleSystem:
         b = L.PolParticleBuffers()
         for name in ("pos", "vel", "radius", "mass", "active", "drag_new", "drag_old", "drag", "u_fluid", "re_p", "cd"):
             setattr(b, name, getattr(self, name).data_ptr())
-        b.force = self.ctx.force.data_ptr()
+        b.drag_field = self.ctx.pdrag.data_ptr()
         b.u_field = self.ctx.u.data_ptr()
         L.check(self.lib.pol_particles_bind(self.ctx.handle, C.byref(self.desc), C.byref(b)), self.ctx.handle,
                 "pol_particles_bind")
@@ -117,17 +117,23 @@ class HipCoffeeParticleSystem:
             return {"active_particles": 0, "avg_reynolds": 0.0, "avg_drag_coeff": 0.0,
                     "max_reaction_force": 0.0, "coupling_quality": "no_particles"}
         re = self.re_p[act].double()
-        fmag = torch.linalg.vector_norm(self.ctx.force.double(), dim=0)
+        dmag = torch.linalg.vector_norm(self.drag_new.double()[:, act], dim=0)
         return {"active_particles": n, "avg_reynolds": float(re.mean()), "max_reynolds": float(re.max()),
-                "avg_drag_coeff": float(self.cd[act].double().mean()), "max_reaction_force": float(fmag.max()),
+                "avg_drag_coeff": float(self.cd[act].double().mean()), "max_reaction_force": float(dmag.max()),
                 "coupling_quality": "active"}
 
-    def momentum_balance(self) -> tuple:
-        """(sum of the reaction force field, -sum of drag_new over active particles), FP64."""
-        field = self.ctx.force.double().sum(dim=(1, 2, 3))
+    def drag_field_sums(self) -> tuple:
+        """FP64 sums of the drag field after couple() -- (sum_c BV_c, sum_c B_c) -- and
+        the same quantities summed over the particles (sum_p beta_p v_p, sum_p beta_p)."""
+        fld = self.ctx.pdrag.double().sum(dim=(1, 2, 3)).cpu().numpy()
         act = self.active.bool()
-        drag = self.drag_new.double()[:, act].sum(dim=1)
-        return field.cpu().numpy(), (-drag).cpu().numpy()
+        rel = self.u_fluid.double() - self.vel.double()
+        mag = torch.linalg.vector_norm(rel, dim=0)
+        beta = torch.where(mag > 1e-8, torch.linalg.vector_norm(self.drag_new.double(), dim=0) / mag.clamp_min(1e-30),
+                           torch.zeros_like(mag))
+        beta = torch.where(act, beta, torch.zeros_like(beta))
+        bv = (beta[None, :] * self.vel.double()).sum(dim=1).cpu().numpy()
+        return fld, np.concatenate([bv, [float(beta.sum())]])
 
 
 def particle_gravity(g: float, density_ratio: float) -> tuple:

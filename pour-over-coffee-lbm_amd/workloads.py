@@ -65,13 +65,13 @@ class V60Workload:
 
 
 def v60_config(n: int = 224, precision: str = "f32", *, les: bool = True, force_field: bool = False,
-               perm_field: bool = False, tau: float = 0.6) -> V60Workload:
+               perm_field: bool = False, particle_field: bool = False, tau: float = 0.6) -> V60Workload:
     params = V60Params() if n == 224 else V60Params.scaled(n)
     geo = build_v60(params)
     cfg = SolverConfig(n, n, n, precision=precision, tau=tau, periodic=(False, False, False), les=les,
                        cs_smag=0.17, nu_sgs_max=0.1, gravity=(0.0, 0.0, -1e-5), darcy_K=1e-3, forch_CF=0.55,
                        inlet_rho=1.0, inlet_u=(0.0, 0.0, -0.03), outlet_rho=1.0, force_field=force_field,
-                       perm_field=perm_field)
+                       perm_field=perm_field, particle_field=particle_field)
     return V60Workload(cfg, geo, 2000)
 
 

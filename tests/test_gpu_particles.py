@@ -48,7 +48,7 @@ def test_particle_index_bit_exact(hip, oracle, dims):
 
 
 def _setup(n, precision, count, seed=WL.SEED):
-    wl = WL.v60_config(n, precision, force_field=True)
+    wl = WL.v60_config(n, precision, particle_field=True)
     seedp = WL.seed_particles(wl.geometry, count, seed=seed)
     return wl, seedp
 
@@ -68,11 +68,11 @@ def _oracle_coupled(oracle, wl, seedp, steps, alpha=0.8):
           "radius": seedp.radius.astype(np.float32).astype(np.float64),
           "mass": seedp.mass.astype(np.float32).astype(np.float64), "active": np.ones(P, np.int32),
           "drag_old": np.zeros((3, P))}
-    force = np.zeros((3, n, n, n))
+    field = np.zeros((4, n, n, n))
     for _ in range(steps):
-        oracle.particles_couple((n, n, n), pp, st, u_field, force)
-        f, rho, u_field, _ = oracle.step(d, f, flags, force=force, want_macro=True)
-        oracle.particles_advance((n, n, n), pp, st, flags, force)
+        oracle.particles_couple((n, n, n), pp, st, u_field, field)
+        f, rho, u_field, _ = oracle.step(d, f, flags, want_macro=True, pdrag=field)
+        oracle.particles_advance((n, n, n), pp, st, flags, field)
     return rho, u_field, st
 
 
@@ -105,8 +105,9 @@ def test_coupled_steps_match_oracle(hip, oracle):
 
 
 def test_momentum_exchange_and_footprint_clear(hip):
-    """Newton's third law in FP64 (field sum = -sum of particle drags) and an
-    all-zero force field after advance (the 8-corner footprint is cleared)."""
+    """The drag-field scatter is a partition of unity (FP64 field sums equal the
+    particle sums of beta v and beta), and the field is all zero after advance
+    (the 8-corner footprint is cleared)."""
     wl, seedp = _setup(64, "f32", 5000)
     ctx = HipContext(wl.config, wl.geometry.flags)
     ctx.init_equilibrium()
@@ -114,13 +115,13 @@ def test_momentum_exchange_and_footprint_clear(hip):
     ps.prime()
     ps.coupled_step(nsteps=30)
     ps.couple()
-    field, drag = ps.momentum_balance()
-    assert np.allclose(field, drag, rtol=1e-5, atol=1e-12)
-    assert np.abs(drag).max() > 0
+    field, parts = ps.drag_field_sums()
+    assert np.allclose(field, parts, rtol=1e-5, atol=1e-12)
+    assert parts[3] > 0
     ctx.step(1, L.POL_WANT_MACRO)
     ps.advance()
     torch.cuda.synchronize()
-    assert float(ctx.force.abs().max()) == 0.0
+    assert float(ctx.pdrag.abs().max()) == 0.0
 
 
 def test_full_size_coupled_run_stays_physical(hip):

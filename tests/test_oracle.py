@@ -296,8 +296,8 @@ def test_trilinear_reference_known_answers(oracle):
           "mass": np.ones(P), "active": np.ones(P, np.int32), "drag_old": np.zeros((3, P))}
     pp = oracle.particle_params(P, rho_fluid=1.0, nu_fluid=0.1, drag_max_per_mass=1e9, alpha=0.8, g=(0, 0, 0),
                                 dt=1.0, max_disp=1.0, wall_damping=0.3)
-    force = np.zeros((3, n, n, n))
-    oracle.particles_couple((n, n, n), pp, st, u_field, force)
+    field = np.zeros((4, n, n, n))
+    oracle.particles_couple((n, n, n), pp, st, u_field, field)
     assert np.allclose(st["u_fluid"], np.array(kat["expected"]).T, atol=1e-12)
 
 
@@ -329,8 +329,8 @@ def test_drag_closed_form_and_newton_third_law(oracle, law):
     corners sums to -F (partition of unity); under-relaxation alpha F_new + (1-alpha) F_old."""
     st, uf, pp = _one_particle(oracle, u=(0.01, -0.02, 0.005))
     pp.drag_law = law
-    force = np.zeros((3, 8, 8, 8))
-    oracle.particles_couple((8, 8, 8), pp, st, uf, force)
+    field = np.zeros((4, 8, 8, 8))
+    oracle.particles_couple((8, 8, 8), pp, st, uf, field)
     rel = np.array([0.01, -0.02, 0.005])
     mag = np.linalg.norm(rel)
     re = mag * 2 * 0.75 / (1 / 30)
@@ -339,9 +339,13 @@ def test_drag_closed_form_and_newton_third_law(oracle, law):
     if law == L.POL_DRAG_TEXTBOOK:
         drag = drag * mag
     assert np.allclose(st["drag_new"][:, 0], drag, rtol=1e-12)
-    assert np.allclose(force.reshape(3, -1).sum(axis=1), -drag, rtol=1e-12)   # partition of unity + reaction
+    # drag field: partition of unity of (w beta v, w beta); beta = |drag| / |u_rel|
+    beta = np.linalg.norm(drag) / mag
+    sums = field.reshape(4, -1).sum(axis=1)
+    assert np.allclose(sums[:3], beta * st["vel"][:, 0], rtol=1e-12, atol=1e-300)
+    assert abs(sums[3] - beta) <= 1e-12 * beta
     assert np.allclose(st["drag"][:, 0], 0.8 * drag, rtol=1e-12)              # alpha F_new + (1-alpha) 0
-    oracle.particles_couple((8, 8, 8), pp, st, uf, force)
+    oracle.particles_couple((8, 8, 8), pp, st, uf, field)
     assert np.allclose(st["drag"][:, 0], 0.8 * drag + 0.2 * 0.8 * drag, rtol=1e-12)
 
 
@@ -351,11 +355,11 @@ def test_settling_velocity(oracle):
     r, m, g = 0.75, 1.2 * math.pi * 1.5**3 / 6, 2e-5
     st, uf, pp = _one_particle(oracle, r=r, m=m, g=(0, 0, -g))
     st["pos"][2] = 6.0
-    force = np.zeros((3, 8, 8, 8))
+    field = np.zeros((4, 8, 8, 8))
     for _ in range(3000):
         st["pos"][:, 0] = [3.3, 3.6, 4.2]      # keep it in place: the fluid field is uniform
-        oracle.particles_couple((8, 8, 8), pp, st, uf, force)
-        oracle.particles_advance((8, 8, 8), pp, st, None, force)
+        oracle.particles_couple((8, 8, 8), pp, st, uf, field)
+        oracle.particles_advance((8, 8, 8), pp, st, None, field)
     v = -st["vel"][2, 0]
 
     def drag(vv):
