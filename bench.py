@@ -27,7 +27,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 BYTES_PER_UPDATE = {"f32": 152, "f16": 76, "bf16": 76, "f64": 304}
-REFERENCE_MLUPS_224 = 159.3    # reference's published 224^3 figure (Apple M-series, Metal)
+# the reference's only published 224^3 figure (Apple M-series, Metal); its own
+# numbers contradict each other, so it is quoted beside the result, not divided by
+REFERENCE_MLUPS_224 = 159.3
 
 
 def log(msg):
@@ -107,31 +109,137 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     return {"bytes_per_copy": nbytes, "copy_ms": round(s * 1e3, 4), "copy_GBs": round(2 * nbytes / s / 1e9, 1)}
 
 
-def cpu_baseline(budget_s=12.0):
-    """The oracle's C restatement (OpenMP, all host threads) on the same V60 224^3
-    workload for a bounded number of steps."""
+def _omp_threads(n):
+    """Set the OpenMP team size of the oracle library (shared libgomp runtime)."""
+    import ctypes
+
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+        return True
+    except OSError:
+        return False
+
+
+def cpu_baseline(budget_s=10.0, single_budget_s=6.0):
+    """The C/OpenMP restatement of the reference CPU backend
+    (cpu_backend.py:79-237: BGK collide + pull stream, FP32, SoA) on the
+    224^3 V60 solid mask, a bounded number of steps, on the host cores; plus
+    a one-thread figure (the reference calls its CPU backend a single-thread
+    reference, cpu_backend.py:36)."""
     import numpy as np
 
     from oracle import oracle as O
     from pourover_amd import workloads as WL
 
     wl = WL.v60_config(224, "f32")
-    d = O.desc_from_dict(wl.config.oracle_dict(True))
-    f = O.init_equilibrium(224, 224, 224, prec="f32")
-    flags = wl.geometry.flags
     counts = wl.geometry.counts()
     active = counts["fluid"] + counts["inlet"] + counts["outlet"]
-    O.step(d, f, flags, prec="f32")  # warm (page-in)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        f, _, _, _ = O.step(d, f, flags, prec="f32")
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 50:
-            break
+    solid = (wl.geometry.kind == 1).astype(np.uint8)
+    n3 = (224, 224, 224)
+    f = np.zeros((19,) + n3, dtype=np.float32)
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float32)
+    f[:] = w[:, None, None, None]
+    f_new = np.empty_like(f)
+    rho = np.empty(n3, dtype=np.float32)
+    u = np.empty((3,) + n3, dtype=np.float32)
+
+    def run(budget, max_steps):
+        O.cpu_backend_step(f, f_new, rho, u, solid, 0.6, prec="f32")  # warm (page-in)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            O.cpu_backend_step(f, f_new, rho, u, solid, 0.6, prec="f32")
+            n += 1
+            el = time.perf_counter() - t0
+            if el > budget or n >= max_steps:
+                return n, el
+
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": active * n / el / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": "%d steps of the 224^3 V60 FP32 workload (oracle_step_f32, OpenMP %d threads)" % (n, threads)}
+    _omp_threads(threads)
+    n, el = run(budget_s, 40)
+    out = {"value": round(active * n / el / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": "%d steps of the 224^3 V60 mask, FP32, C/OpenMP restatement of the reference CPU backend "
+                     "(collide + pull stream), %d threads; MLUPS over the %d non-solid cells" % (n, threads, active),
+           "all_site_mlups": round(224**3 * n / el / 1e6, 2)}
+    if _omp_threads(1):
+        n1, el1 = run(single_budget_s, 4)
+        out["single_thread"] = {"value": round(active * n1 / el1 / 1e6, 3), "cores": 1, "steps": n1}
+        _omp_threads(threads)
+    # the same V60 physics as the GPU step (oracle_step_f32: walls, porous
+    # layer, LES, Guo gravity, inlet/outlet), same threads
+    d = O.desc_from_dict(wl.config.oracle_dict(True))
+    g = O.init_equilibrium(224, 224, 224, prec="f32")
+    flags = wl.geometry.flags
+    g, _, _, _ = O.step(d, g, flags, prec="f32")
+    n2, t0 = 0, time.perf_counter()
+    while True:
+        g, _, _, _ = O.step(d, g, flags, prec="f32")
+        n2 += 1
+        el2 = time.perf_counter() - t0
+        if el2 > budget_s or n2 >= 40:
+            break
+    out["full_physics_port"] = {"value": round(active * n2 / el2 / 1e6, 2), "cores": threads, "steps": n2,
+                                "sample": "oracle_step_f32 on the V60 workload (the GPU step's physics)"}
+    return out
+
+
+def bench_adjoint(n=128, steps=100, reps=20):
+    """Adjoint cost at the differentiation workload's grid (scaled V60, FP32):
+    kernel time of one reverse step (forward recompute of the state + adjoint
+    kernel) over one forward step, both from HIP events on the launch stream,
+    and the whole checkpointed gradient over a plain forward run (wall)."""
+    import torch
+
+    from pourover_amd import _lib as L
+    from pourover_amd import workloads as WL
+    from pourover_amd.adjoint import AdjointProblem
+
+    wl = WL.v60_config(n, "f32", perm_field=True)
+    K = WL.permeability_field(wl.geometry)
+    prob = AdjointProblem(wl, K, steps=steps, window=20, checkpoint=50)
+    ctx, lib = prob.ctx, prob.ctx.lib
+    ctx.init_equilibrium()
+    ctx.step(20)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    a, b, c, d = ev(), ev(), ev(), ev()
+    torch.cuda.synchronize()
+    a.record()
+    ctx.step(reps)
+    b.record()
+    state = prob.seg[0]
+    state.copy_(ctx.f_current)
+    lib.pol_adjoint_reset(ctx.handle, L.stream_handle())
+    torch.cuda.synchronize()
+    c.record()
+    for _ in range(reps):
+        st = lib.pol_adjoint_step(ctx.handle, L.ptr(state), 1, L.stream_handle())
+        L.check(st, ctx.handle, "pol_adjoint_step")
+    d.record()
+    torch.cuda.synchronize()
+    fwd_ms = a.elapsed_time(b) / reps
+    adj_ms = c.elapsed_time(d) / reps
+    J, _, _ = prob.gradient()
+    t_grad = prob.timings["forward_s"] + prob.timings["reverse_s"]
+    t_fwd = prob.forward_time()
+    out = {"grid": [n, n, n], "precision": "f32", "forward_kernel_ms": round(fwd_ms, 4),
+           "adjoint_kernel_ms": round(adj_ms, 4), "adjoint_step_over_forward": round((fwd_ms + adj_ms) / fwd_ms, 3),
+           "adjoint_kernel_over_forward": round(adj_ms / fwd_ms, 3),
+           "gradient_over_forward_wall": round(t_grad / t_fwd, 2), "steps": steps, "checkpoint_every": 50}
+    del prob
+    torch.cuda.empty_cache()
+    return out
+
+
+def counter_traffic(precision):
+    """HBM bytes per launch of the stream-collide kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
+    1 GiB device copy reads for its known bytes), or None."""
+    path = os.path.join(ROOT, "profiles", "stream_collide_v60_224_%s_counters.md" % precision)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def main():
@@ -175,7 +283,7 @@ def main():
         "ms_per_step": round(r["wall_s"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(mlups / REFERENCE_MLUPS_224, 2),
+        "vs_baseline": None,
         "dtype": prec,
         "data": "synthetic (V60 geometry generated in-tree, fluid at rest + inlet jet)",
         "config": {"workload": "v60_224_%s" % prec, "grid": [224, 224, 224], "active_cells": r["active_cells"],
@@ -183,9 +291,11 @@ def main():
                    "les_cs": 0.17, "gravity": 1e-5, "darcy_K": 1e-3, "forch_CF": 0.55, "inlet_u": -0.03,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counter_traffic(prec),
                      "algorithmic_bytes_per_launch": algo_bytes, "kernel_ms": round(k_s * 1e3, 4)},
         "all_site_mlups": round(r["cells"] / k_s / 1e6, 1),
+        "reference_published": {"mlups_224": REFERENCE_MLUPS_224, "hardware": "Apple M1/M2 Pro, Metal",
+                                "note": "self-inconsistent prose figure, all-site count"},
     }
     if not args.quick:
         log("bench: periodic 224^3 box f32")
@@ -201,6 +311,8 @@ def main():
             out["%s_storage" % p16] = {"mlups": round(r16["active_cells"] / s16 / 1e6, 1),
                                        "kernel_ms": round(s16 * 1e3, 4), "speedup_vs_f32": round(k_s / s16, 3),
                                        "achieved_GBs": round(76 * r16["active_cells"] / s16 / 1e9, 1)}
+        log("bench: adjoint 128^3 f32")
+        out["adjoint"] = bench_adjoint()
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

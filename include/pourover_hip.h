@@ -251,6 +251,9 @@ int pol_particle_index(const float *pos, int32_t n, int32_t nx, int32_t ny, int3
                        int32_t *idx, float *frac, void *stream);
 
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */
+#define POL_RFLAG_CLAMP 0x80000000u /* rev_flags bit 31: cell lies on a non-periodic face
+                                       (clamped readers of its populations possible) */
+
 typedef struct pol_adjoint_buffers {
   void *lam[2];        /* adjoint populations [19][cells], real type */
   double *grad_perm;   /* [cells] accumulated dJ/dK (FP64), may be NULL */

@@ -19,6 +19,7 @@ LIB_PATH = PKG_DIR / "libpourover_hip.so"
 POL_ABI_VERSION = 1
 POL_Q = 19
 POL_NRED = 12
+POL_CHUNK = 32  # cells per launch chunk
 POL_OK, POL_ERR_INVALID, POL_ERR_HIP, POL_ERR_ALLOC, POL_ERR_NO_DEVICE, POL_ERR_STATE, POL_ERR_UNSUPPORTED = range(7)
 POL_F32, POL_F16, POL_BF16, POL_F64 = range(4)
 POL_PERIODIC_X = 0x1

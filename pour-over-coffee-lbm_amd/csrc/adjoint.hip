@@ -27,7 +27,6 @@ namespace pol {
 // reverse-gather flag word (adjoint only, built on the host with the geometry)
 //   bit p (0..18): the normal reader y + e_p of population p exists and gathers
 //   bit 31       : y lies on a non-periodic face where clamped readers may exist
-#define POL_RFLAG_CLAMP 0x80000000u
 
 template <typename R>
 __device__ inline void eq_grad_u(R rho, R ux, R uy, R uz, const R (&lam)[Q], R& ax, R& ay, R& az) {

@@ -106,13 +106,32 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     const bool les = FULL && (a.options & POL_LES) && !(fl & POL_FLAG_NO_LES);
     if constexpr (STORE) {
       o = collide_cell<R, FULL>(f, kind, porous, les, K, fx, fy, fz, bx, by, bz, B, P);
+    } else {
+      // moments only: the same velocity definition the collision uses
+      o = collide_cell<R, FULL>(f, kind, porous, false, K, fx, fy, fz, bx, by, bz, B, P);
+    }
+  }
+  if constexpr (STORE) {
+    // Solid lanes next to fluid ones store zero deviations (f = w; no gather
+    // ever reads a solid cell), so that every 128-byte line of the output is
+    // written whole: a partly written line costs HBM a read-modify-write.
+    // Chunk launches list only chunks holding fluid, so all their lanes store;
+    // dense launches store a solid lane when its 32-lane group holds fluid.
+    bool store = active;
+    if constexpr (FLAGS) {
+      const uint64_t any = __ballot(active);
+      const uint64_t group = 0xffffffffull << (threadIdx.x & 32);
+      store = inside && (a.chunks != nullptr || (any & group) != 0);
+      if (!active) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = R(0);
+      }
+    }
+    if (store) {
       static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         C::store(fout + (size_t)q * N + cell, f[q]);
       });
-    } else {
-      // moments only: the same velocity definition the collision uses
-      o = collide_cell<R, FULL>(f, kind, porous, false, K, fx, fy, fz, bx, by, bz, B, P);
     }
   }
 

@@ -14,7 +14,7 @@ import torch
 
 from . import _lib as L
 
-POL_CHUNK = 32  # cells per launch chunk (include/pourover_hip.h)
+POL_CHUNK = L.POL_CHUNK
 _STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 
 
@@ -146,10 +146,12 @@ class HipContext:
         L.check(self.lib.pol_bind(self._ctx, C.byref(b)), self._ctx, "pol_bind")
 
     def _chunk_list(self, flags: Optional[np.ndarray]) -> Optional[torch.Tensor]:
-        """Launch list: first cells of the POL_CHUNK-cell, line-aligned row chunks
-        holding any non-solid cell (None when every chunk does or there are no
-        flags).  Solid cells are never read or written, so chunks of solid cells
-        get no threads."""
+        """Launch list: first cells of the POL_CHUNK-cell row chunks holding any
+        non-solid cell (None when every chunk does or there are no flags).
+        Chunks of solid cells get no threads; solid lanes of a listed chunk
+        store zero deviations so output lines are written whole.  With 16-bit
+        storage a 128-byte line spans two chunks, so both chunks of a line are
+        listed when either holds fluid (rows of a multiple of POL_CHUNK cells)."""
         if flags is None:
             return None
         cfg = self.config
@@ -160,6 +162,10 @@ class HipContext:
         if pad:
             open_cell = np.concatenate([open_cell, np.zeros(open_cell.shape[:2] + (pad,), bool)], axis=2)
         live = open_cell.reshape(cfg.nz, cfg.ny, nch, POL_CHUNK).any(axis=3)
+        per_line = 128 // (cfg.bytes_per_population * POL_CHUNK)
+        if not pad and per_line > 1 and live.size % per_line == 0:
+            grp = live.reshape(-1, per_line).any(axis=1, keepdims=True)
+            live = np.broadcast_to(grp, (grp.shape[0], per_line)).reshape(live.shape)
         zz, yy, cc = np.nonzero(live)
         if zz.size == 0 or zz.size == live.size:
             return None

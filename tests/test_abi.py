@@ -1,0 +1,142 @@
+"""The C ABI without a GPU: the library loads, exports every function that
+include/pourover_hip.h declares, and the ctypes mirror (_lib.py) agrees with
+the header on constants and struct layouts (offsets from a gcc-compiled probe
+of the header itself).  No compute entry point is called."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from pourover_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "pourover_hip.h"
+
+
+def _header_text() -> str:
+    txt = HEADER.read_text()
+    txt = re.sub(r"/\*.*?\*/", " ", txt, flags=re.S)
+    return txt
+
+
+def _declared_functions() -> list[str]:
+    return sorted(set(re.findall(r"\b(pol_[a-z0-9_]+)\s*\(", _header_text())))
+
+
+def _defines() -> dict[str, int]:
+    out = {}
+    for name, val in re.findall(r"#define\s+(POL_[A-Z0-9_]+)\s+(0x[0-9A-Fa-f]+|\d+)u?\b", _header_text()):
+        out[name] = int(val, 0)
+    return out
+
+
+def _enums() -> dict[str, int]:
+    out = {}
+    for body in re.findall(r"enum\s*\{(.*?)\}", _header_text(), flags=re.S):
+        nxt = 0
+        for item in body.split(","):
+            item = item.strip()
+            if not item:
+                continue
+            if "=" in item:
+                k, v = (s.strip() for s in item.split("="))
+                nxt = int(v, 0)
+            else:
+                k = item
+            out[k] = nxt
+            nxt += 1
+    return out
+
+
+def test_library_loads_and_exports_every_declared_function():
+    lib = L.load()
+    decl = _declared_functions()
+    assert len(decl) >= 20
+    missing = [n for n in decl if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.pol_version() == L.POL_ABI_VERSION
+
+
+def test_ctypes_signatures_cover_the_header():
+    assert sorted(L.SIGNATURES) == _declared_functions()
+
+
+def test_status_strings_need_no_device():
+    lib = L.load()
+    for st in range(7):
+        assert lib.pol_status_string(st)
+    # argument validation happens before any HIP call
+    assert lib.pol_create(None, None) == L.POL_ERR_INVALID
+
+
+def test_constants_match_header():
+    d = _defines()
+    e = _enums()
+    for name, val in list(d.items()) + list(e.items()):
+        if name in ("POUROVER_HIP_H",):
+            continue
+        if name.startswith("POL_RED_"):
+            continue  # slots: checked below against RED_SLOTS
+        assert hasattr(L, name), name
+        assert getattr(L, name) == val, (name, getattr(L, name), val)
+    red = sorted((v, k) for k, v in d.items() if k.startswith("POL_RED_"))
+    assert [v for v, _ in red] == list(range(L.POL_NRED)) == list(range(len(L.RED_SLOTS)))
+
+
+_PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "pourover_hip.h"
+#define F(T, m) printf(#T " " #m " %zu %zu\n", offsetof(T, m), sizeof(((T *)0)->m));
+#define S(T) printf(#T " __size__ %zu %zu\n", sizeof(T), (size_t)0);
+int main(void) {
+  S(pol_desc) F(pol_desc, nx) F(pol_desc, ny) F(pol_desc, nz) F(pol_desc, storage) F(pol_desc, options)
+  F(pol_desc, device) F(pol_desc, tau) F(pol_desc, cs_smag) F(pol_desc, nu_sgs_max) F(pol_desc, gravity)
+  F(pol_desc, darcy_K) F(pol_desc, forch_CF) F(pol_desc, inlet_rho) F(pol_desc, inlet_u) F(pol_desc, outlet_rho)
+  S(pol_buffers) F(pol_buffers, f) F(pol_buffers, cell_flags) F(pol_buffers, force) F(pol_buffers, perm)
+  F(pol_buffers, rho) F(pol_buffers, u) F(pol_buffers, active_chunks) F(pol_buffers, n_chunks)
+  F(pol_buffers, reserved) F(pol_buffers, particle_drag)
+  S(pol_particle_desc) F(pol_particle_desc, count) F(pol_particle_desc, drag_law) F(pol_particle_desc, rho_fluid)
+  F(pol_particle_desc, nu_fluid) F(pol_particle_desc, drag_max_per_mass) F(pol_particle_desc, relax_alpha)
+  F(pol_particle_desc, gravity) F(pol_particle_desc, dt) F(pol_particle_desc, max_disp)
+  F(pol_particle_desc, wall_damping)
+  S(pol_particle_buffers) F(pol_particle_buffers, pos) F(pol_particle_buffers, vel) F(pol_particle_buffers, radius)
+  F(pol_particle_buffers, mass) F(pol_particle_buffers, active) F(pol_particle_buffers, drag_new)
+  F(pol_particle_buffers, drag_old) F(pol_particle_buffers, drag) F(pol_particle_buffers, u_fluid)
+  F(pol_particle_buffers, re_p) F(pol_particle_buffers, cd) F(pol_particle_buffers, drag_field)
+  F(pol_particle_buffers, u_field)
+  S(pol_adjoint_buffers) F(pol_adjoint_buffers, lam) F(pol_adjoint_buffers, grad_perm)
+  F(pol_adjoint_buffers, rev_flags)
+  return 0;
+}
+"""
+
+_CTYPES = {"pol_desc": L.PolDesc, "pol_buffers": L.PolBuffers, "pol_particle_desc": L.PolParticleDesc,
+           "pol_particle_buffers": L.PolParticleBuffers, "pol_adjoint_buffers": L.PolAdjointBuffers}
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+def test_struct_layouts_match_header(tmp_path):
+    src = tmp_path / "probe.c"
+    exe = tmp_path / "probe"
+    src.write_text(_PROBE)
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, stdout=subprocess.PIPE, text=True).stdout.split("\n")
+    seen = {k: set() for k in _CTYPES}
+    for line in filter(None, out):
+        tname, field, off, size = line.split()
+        ct = _CTYPES[tname]
+        if field == "__size__":
+            assert C.sizeof(ct) == int(off), tname
+            continue
+        desc = getattr(ct, field)
+        assert desc.offset == int(off), (tname, field, desc.offset, off)
+        assert desc.size == int(size), (tname, field, desc.size, size)
+        seen[tname].add(field)
+    for tname, ct in _CTYPES.items():
+        assert seen[tname] == {f[0] for f in ct._fields_}, tname
