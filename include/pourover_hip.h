@@ -121,14 +121,15 @@ typedef struct pol_buffers {
   void *perm;           /* real [cells] permeability (POL_PERM_FIELD) */
   void *rho;            /* real [cells] macroscopic output, may be NULL */
   void *u;              /* real [3][cells] macroscopic output, may be NULL */
-  const uint32_t *active_chunks; /* ascending first-cell indices of the
-                                    POL_CHUNK-cell row chunks (x0 % POL_CHUNK
-                                    == 0) that hold at least one non-solid
-                                    cell, or NULL to launch over every cell;
-                                    with a list the kernels touch only those
-                                    chunks (solid lanes idle) */
+  const uint32_t *active_chunks; /* first-cell indices of the chunk_cells-cell
+                                    row chunks (x0 % chunk_cells == 0) that
+                                    hold at least one non-solid cell, or NULL
+                                    to launch over every cell.  With a list
+                                    the kernels touch only those chunks; their
+                                    solid lanes store zero deviations (f = w),
+                                    so written lines are whole */
   uint32_t n_chunks;
-  uint32_t reserved;
+  uint32_t chunk_cells; /* cells per chunk: 8, 16, 32 or 64 (0 = POL_CHUNK) */
   void *particle_drag; /* real [4][cells] (POL_PARTICLE_FIELD): per cell
                           sum_p w beta_p v_p (x, y, z) and B = sum_p w beta_p
                           over the particles whose trilinear footprint covers
@@ -136,7 +137,7 @@ typedef struct pol_buffers {
                           The collision treats the drag B (V - u) implicitly in
                           u, like the Darcy term, so any B is stable. */
 } pol_buffers;
-#define POL_CHUNK 32 /* one 128-byte line of FP32 populations */
+#define POL_CHUNK 32 /* default chunk_cells */
 
 /* ---- library ------------------------------------------------------------ */
 /* ABI version (POL_ABI_VERSION). */

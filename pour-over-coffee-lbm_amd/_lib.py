@@ -62,7 +62,7 @@ class PolBuffers(C.Structure):
     _fields_ = [
         ("f", C.c_void_p * 2), ("cell_flags", C.c_void_p), ("force", C.c_void_p),
         ("perm", C.c_void_p), ("rho", C.c_void_p), ("u", C.c_void_p),
-        ("active_chunks", C.c_void_p), ("n_chunks", C.c_uint32), ("reserved", C.c_uint32),
+        ("active_chunks", C.c_void_p), ("n_chunks", C.c_uint32), ("chunk_cells", C.c_uint32),
         ("particle_drag", C.c_void_p),
     ]
 

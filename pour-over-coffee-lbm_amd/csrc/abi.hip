@@ -18,6 +18,8 @@ struct pol_ctx {
   uint32_t N = 0;
   Grid grid{};   // dense grid over all cells (sizes the partial buffers)
   Grid wgrid{};  // work grid: over the active-cell list when bound, else = grid
+  uint32_t cshift = 5;    // log2 of the cells per chunk of the bound list
+  uint32_t part_cap = 0;  // blocks the partial buffers hold (any work grid fits)
   double* partial = nullptr;  // [nblk][POL_NRED]
   double* red_dev = nullptr;  // [POL_NRED]
   // particles
@@ -98,16 +100,18 @@ int pol_create(pol_ctx** out, const pol_desc* desc) {
   c->N = (uint32_t)N;
   c->grid = make_grid(N);
   c->wgrid = c->grid;
+  // the largest work grid: every row chunk listed, rows padded to 64 cells
+  c->part_cap = make_grid((uint64_t)d.ny * d.nz * ((uint64_t)d.nx + 63)).nblk;
   hipError_t e = hipSetDevice(d.device);
-  if (e == hipSuccess) e = hipMalloc((void**)&c->partial, (size_t)c->grid.nblk * POL_NRED * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->partial, (size_t)c->part_cap * POL_NRED * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->red_dev, POL_NRED * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc((void**)&c->adj_partial, (size_t)c->grid.nblk * 4 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->adj_partial, (size_t)c->part_cap * 4 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->adj_acc, 4 * sizeof(double));
   if (e != hipSuccess) {
-    hipFree(c->partial);
-    hipFree(c->red_dev);
-    hipFree(c->adj_partial);
-    hipFree(c->adj_acc);
+    (void)hipFree(c->partial);
+    (void)hipFree(c->red_dev);
+    (void)hipFree(c->adj_partial);
+    (void)hipFree(c->adj_acc);
     delete c;
     return POL_ERR_ALLOC;
   }
@@ -117,11 +121,11 @@ int pol_create(pol_ctx** out, const pol_desc* desc) {
 
 int pol_destroy(pol_ctx* c) {
   if (!c) return POL_OK;
-  hipSetDevice(c->d.device);
-  hipFree(c->partial);
-  hipFree(c->red_dev);
-  hipFree(c->adj_partial);
-  hipFree(c->adj_acc);
+  (void)hipSetDevice(c->d.device);
+  (void)hipFree(c->partial);
+  (void)hipFree(c->red_dev);
+  (void)hipFree(c->adj_partial);
+  (void)hipFree(c->adj_acc);
   delete c;
   return POL_OK;
 }
@@ -138,13 +142,23 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
   if ((o & POL_PARTICLE_FIELD) && !b->particle_drag)
     return fail(c, POL_ERR_INVALID, "pol_bind: POL_PARTICLE_FIELD without particle_drag buffer");
   if ((o & POL_PERM_FIELD) && !(o & POL_USE_FLAGS)) return fail(c, POL_ERR_INVALID, "pol_bind: permeability needs cell flags");
-  if (b->active_chunks && (b->n_chunks == 0 || (uint64_t)b->n_chunks * POL_CHUNK > (uint64_t)c->N + POL_CHUNK))
+  const uint32_t cc = b->chunk_cells ? b->chunk_cells : POL_CHUNK;
+  if (cc != 8 && cc != 16 && cc != 32 && cc != 64)
+    return fail(c, POL_ERR_INVALID, "pol_bind: chunk_cells must be 8, 16, 32 or 64");
+  const uint32_t cshift = cc == 8 ? 3 : (cc == 16 ? 4 : (cc == 32 ? 5 : 6));
+  const uint64_t max_chunks = (uint64_t)c->d.ny * c->d.nz * (((uint64_t)c->d.nx + cc - 1) / cc);
+  if (b->active_chunks &&
+      (b->n_chunks == 0 || b->n_chunks > max_chunks || ((uint64_t)b->n_chunks << cshift) >= (1ull << 32)))
     return fail(c, POL_ERR_INVALID, "pol_bind: n_chunks out of range");
   if (b->active_chunks && !(o & POL_USE_FLAGS))
     return fail(c, POL_ERR_INVALID, "pol_bind: a chunk list needs cell flags (solid lanes are masked by them)");
+  const Grid wg = b->active_chunks ? make_grid((uint64_t)b->n_chunks << cshift) : c->grid;
+  if (wg.nblk > c->part_cap) return fail(c, POL_ERR_INVALID, "pol_bind: work grid exceeds the reduction scratch");
   c->b = *b;
+  c->b.chunk_cells = cc;
+  c->cshift = cshift;
   if (!(o & POL_USE_FLAGS)) c->b.cell_flags = nullptr;
-  c->wgrid = c->b.active_chunks ? make_grid((uint64_t)c->b.n_chunks * POL_CHUNK) : c->grid;
+  c->wgrid = wg;
   c->bound = true;
   c->cur = 0;
   c->err.clear();
@@ -201,7 +215,8 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.dnx = make_fastdiv(d.nx);
   a.dny = make_fastdiv(d.ny);
   a.chunks = c->b.active_chunks;
-  a.n_items = c->b.active_chunks ? c->b.n_chunks * POL_CHUNK : c->N;
+  a.n_items = c->b.active_chunks ? c->b.n_chunks << c->cshift : c->N;
+  a.cshift = c->cshift;
   a.per_xcd = c->wgrid.per_xcd;
   a.nblk = c->wgrid.nblk;
   a.options = d.options;

@@ -35,11 +35,11 @@ __device__ inline void eq_grad_u(R rho, R ux, R uy, R uz, const R (&lam)[Q], R& 
   static_for<0, Q>([&](auto qc) {
     constexpr int p = decltype(qc)::value;
     const R lw = lam[p] * W<R>(p);
-    const R eu = R(EX(p)) * ux + R(EY(p)) * uy + R(EZ(p)) * uz;
+    const R eu = edot<EX(p), EY(p), EZ(p)>(ux, uy, uz);
     const R c = R(3) * lw + R(9) * lw * eu;
-    sx += R(EX(p)) * c;
-    sy += R(EY(p)) * c;
-    sz += R(EZ(p)) * c;
+    eacc<EX(p)>(sx, c);
+    eacc<EY(p)>(sy, c);
+    eacc<EZ(p)>(sz, c);
     s0 += lw;
   });
   ax = rho * (sx - R(3) * ux * s0);
@@ -52,14 +52,16 @@ struct AdjGeom {
   bool px, py, pz;
 };
 
-template <typename S, typename R>
+template <typename S, typename R, bool A32>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const uint32_t* __restrict__ rflags) {
   using Cd = Codec<S>;
+  using AD = Addr<A32>;
+  using AI = typename AD::idx;
   const StepArgs& a = A.fw;
   const uint32_t lblk = logical_block(a.per_xcd);
   if (lblk >= a.nblk) return;
   uint32_t cell, yz, x;
-  const bool inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.dnx, a.nx, cell, yz, x);
+  const bool inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
   const uint32_t N = a.N;
   const uint32_t z = fdiv(yz, a.dny);
   const uint32_t y = yz - z * a.ny;
@@ -89,9 +91,9 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       const int off = (EX(p) > 0 ? oxp : (EX(p) < 0 ? oxm : 0)) + (EY(p) > 0 ? oyp : (EY(p) < 0 ? oym : 0)) +
                       (EZ(p) > 0 ? ozp : (EZ(p) < 0 ? ozm : 0));
       R v = R(0);
-      if (rf & (1u << p)) v = mu_in[(size_t)p * N + (uint32_t)((int)cell + off)];
+      if (rf & (1u << p)) v = *AD::at(mu_in, (AI)p * N + (uint32_t)((int)cell + off));
       if constexpr (p != 0) {
-        if (kind != POL_KIND_INLET && (fl & (1u << OPP(p)))) v += mu_in[(size_t)OPP(p) * N + cell];
+        if (kind != POL_KIND_INLET && (fl & (1u << OPP(p)))) v += *AD::at(mu_in, (AI)OPP(p) * N + cell);
       }
       lam[p] = v;
     });
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           const uint32_t rfl = a.flags[rc];
           const uint32_t rk = (rfl >> POL_KIND_SHIFT) & POL_KIND_MASK;
           if (rk == POL_KIND_SOLID || rk == POL_KIND_INLET || (rfl & (1u << p))) continue;
-          lam[p] += mu_in[(size_t)p * N + rc];
+          lam[p] += *AD::at(mu_in, (AI)p * N + rc);
         }
       });
     }
@@ -142,25 +144,45 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       // nothing gathered: mu at inlet cells is never read
     } else {
       const S* __restrict__ fin = (const S*)a.fin;
-      R g[Q];
+      // Stream the gathered cell's stored deviations D_q = 36 (g_q - w_q)
+      // once, reducing them on the fly to what the adjoint needs: moments,
+      // the second moment (LES stress) and sum_q lam_q D_q.  No D array stays
+      // live beside lam.
+      R s36 = 0, mx = 0, my = 0, mz = 0, lamD = 0;
+      R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
       static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
                         (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
-        size_t src = (size_t)q * N + (uint32_t)((int)cell + off);
+        AI src = (AI)q * N + (uint32_t)((int)cell + off);
         if constexpr (q != 0) {
-          if (fl & (1u << q)) src = (size_t)OPP(q) * N + cell;
+          if (fl & (1u << q)) src = (AI)OPP(q) * N + cell;
         }
-        g[q] = Cd::load(fin + src) / R(36) + W<R>(q);  // raw population for the adjoint algebra
+        const R d = Cd::load(AD::at(fin, src));
+        s36 += d;
+        lamD += lam[q] * d;
+        if constexpr (EX(q) > 0) mx += d;
+        if constexpr (EX(q) < 0) mx -= d;
+        if constexpr (EY(q) > 0) my += d;
+        if constexpr (EY(q) < 0) my -= d;
+        if constexpr (EZ(q) > 0) mz += d;
+        if constexpr (EZ(q) < 0) mz -= d;
+        if constexpr (EX(q) != 0) pxx += d;
+        if constexpr (EY(q) != 0) pyy += d;
+        if constexpr (EZ(q) != 0) pzz += d;
+        if constexpr (EX(q) * EY(q) > 0) pxy += d;
+        if constexpr (EX(q) * EY(q) < 0) pxy -= d;
+        if constexpr (EX(q) * EZ(q) > 0) pxz += d;
+        if constexpr (EX(q) * EZ(q) < 0) pxz -= d;
+        if constexpr (EY(q) * EZ(q) > 0) pyz += d;
+        if constexpr (EY(q) * EZ(q) < 0) pyz -= d;
       });
-      R rho = 0, mx = 0, my = 0, mz = 0;
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        rho += g[q];
-        mx += R(EX(q)) * g[q];
-        my += R(EY(q)) * g[q];
-        mz += R(EZ(q)) * g[q];
-      });
+      // the linearisation point of the adjoint: a rounded 1/36 is immaterial
+      const R i36 = R(1.0 / 36.0);
+      const R rd = s36 * i36, rho = R(1) + rd;
+      mx *= i36;
+      my *= i36;
+      mz *= i36;
       const R inv_rho = R(1) / rho;
       R F0x = G0, F0y = G1, F0z = G2;
       if (a.options & POL_FORCE_FIELD) {
@@ -171,15 +193,16 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       }
       const R vx = (mx + R(0.5) * F0x) * inv_rho, vy = (my + R(0.5) * F0y) * inv_rho,
               vz = (mz + R(0.5) * F0z) * inv_rho;
-      R mu[Q];
+      auto put = [&](int q, R v) { *AD::at(mu_out, (AI)q * N + cell) = v; };
       if (kind == POL_KIND_OUTLET) {
         const R ro = (R)a.outlet_rho;
         R ax, ay, az;
         eq_grad_u<R>(ro, vx, vy, vz, lam, ax, ay, az);
         static_for<0, Q>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
-          mu[q] = (ax * (R(EX(q)) - vx) + ay * (R(EY(q)) - vy) + az * (R(EZ(q)) - vz)) * inv_rho;
-          if (A.objective) mu[q] -= R(EZ(q));
+          R m = (ax * (R(EX(q)) - vx) + ay * (R(EY(q)) - vy) + az * (R(EZ(q)) - vz)) * inv_rho;
+          if (A.objective) m -= R(EZ(q));
+          put(q, m);
         });
         if (A.objective) part[3] = -(double)(mz + R(0.5) * F0z);
       } else {
@@ -211,28 +234,17 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           Fz -= cc * uz;
         }
         const R usq = ux * ux + uy * uy + uz * uz;
-        R fe[Q];
-        static_for<0, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          fe[q] = feq_q<R>(EX(q), EY(q), EZ(q), W<R>(q), rho, ux, uy, uz, usq);
-        });
         R tau = tau0, Qn = 0, sroot = 0;
-        R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
         bool les_active = false;
         if (les) {
-          static_for<1, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const R d = g[q] - fe[q];
-            if constexpr (EX(q) != 0) pxx += d;
-            if constexpr (EY(q) != 0) pyy += d;
-            if constexpr (EZ(q) != 0) pzz += d;
-            if constexpr (EX(q) * EY(q) > 0) pxy += d;
-            if constexpr (EX(q) * EY(q) < 0) pxy -= d;
-            if constexpr (EX(q) * EZ(q) > 0) pxz += d;
-            if constexpr (EX(q) * EZ(q) < 0) pxz -= d;
-            if constexpr (EY(q) * EZ(q) > 0) pyz += d;
-            if constexpr (EY(q) * EZ(q) < 0) pyz -= d;
-          });
+          // Pi_ab = sum_q e_a e_b (g_q - feq_q) = sum_q e_a e_b D_q / 36 - rd/3 d_ab - rho u_a u_b
+          const R r3 = rd * R(1.0 / 3.0);
+          pxx = pxx * i36 - r3 - rho * ux * ux;
+          pyy = pyy * i36 - r3 - rho * uy * uy;
+          pzz = pzz * i36 - r3 - rho * uz * uz;
+          pxy = pxy * i36 - rho * ux * uy;
+          pxz = pxz * i36 - rho * ux * uz;
+          pyz = pyz * i36 - rho * uy * uz;
           Qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
           sroot = sqrt(tau0 * tau0 + (R)a.les_c * Qn * inv_rho);
           tau = R(0.5) * (tau0 + sroot);
@@ -254,20 +266,27 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         static_for<0, Q>([&](auto qc) {
           constexpr int p = decltype(qc)::value;
           const R lw = lam[p] * W<R>(p);
-          const R eu = R(EX(p)) * ux + R(EY(p)) * uy + R(EZ(p)) * uz;
-          const R eF = R(EX(p)) * Fx + R(EY(p)) * Fy + R(EZ(p)) * Fz;
-          Arho += lam[p] * fe[p];
+          const R eu = edot<EX(p), EY(p), EZ(p)>(ux, uy, uz);
+          const R eF = edot<EX(p), EY(p), EZ(p)>(Fx, Fy, Fz);
+          // feq_p - w_p = w_p (rd + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u))
+          const R fdev = W<R>(p) * (rd + rho * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
+          Arho += lam[p] * fdev;
           Sw += lw;
-          const R e3[3] = {R(EX(p)), R(EY(p)), R(EZ(p))};
-          for (int d = 0; d < 3; ++d) {
-            lwe[d] += lw * e3[d];
-            lweu[d] += lw * eu * e3[d];
-            lweF[d] += lw * eF * e3[d];
-          }
+          const R lweu_p = lw * eu, lweF_p = lw * eF;
+          eacc<EX(p)>(lwe[0], lw);
+          eacc<EY(p)>(lwe[1], lw);
+          eacc<EZ(p)>(lwe[2], lw);
+          eacc<EX(p)>(lweu[0], lweu_p);
+          eacc<EY(p)>(lweu[1], lweu_p);
+          eacc<EZ(p)>(lweu[2], lweu_p);
+          eacc<EX(p)>(lweF[0], lweF_p);
+          eacc<EY(p)>(lweF[1], lweF_p);
+          eacc<EZ(p)>(lweF[2], lweF_p);
           // d f*_p / d omega = fe_p - g_p - S_p / (2 cf)
-          domega += lam[p] * (fe[p] - g[p]) - R(0.5) * lw * (R(3) * (eF - uF) + R(9) * eu * eF);
+          domega += lam[p] * fdev - R(0.5) * lw * (R(3) * (eF - uF) + R(9) * eu * eF);
         });
-        Arho *= inv_rho;
+        domega -= lamD * i36;  // the -g_p part of fe_p - g_p, less the w_p in both
+        Arho = (Arho + Sw) * inv_rho;  // sum lam feq / rho with feq = w + fdev
         const R u3[3] = {ux, uy, uz}, F3[3] = {Fx, Fy, Fz};
         R Au[3], Bu[3], CF3[3];
         for (int d = 0; d < 3; ++d) {
@@ -280,10 +299,11 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           R gu[3];
           for (int d = 0; d < 3; ++d) gu[d] = om * Au[d] + Bu[d];
           const R base = om * Arho;
-          R Lam = 0, trP3 = 0;
+          R Lam = 0, trP3 = 0, iQn = 0;
           if (les_active) {
             Lam = domega * (-om * om) * (R)a.les_c / (R(4) * sroot * rho);
-            trP3 = (pxx + pyy + pzz) / R(3);
+            trP3 = (pxx + pyy + pzz) * R(1.0 / 3.0);
+            iQn = R(1) / Qn;
           }
           static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -292,9 +312,9 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
             if (les_active) {
               const R wPw = wx * wx * pxx + wy * wy * pyy + wz * wz * pzz +
                             R(2) * (wx * wy * pxy + wx * wz * pxz + wy * wz * pyz);
-              m += Lam * ((wPw - trP3) / Qn - Qn * inv_rho);
+              m += Lam * ((wPw - trP3) * iQn - Qn * inv_rho);
             }
-            mu[q] = m;
+            put(q, m);
           });
         } else {
           // F = F0 - rho (dk + fk |u|) u ; u = v s(|v|)
@@ -319,8 +339,8 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           }
           static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            mu[q] = (R(1) - om) * lam[q] + grho +
-                    (vb[0] * (R(EX(q)) - vx) + vb[1] * (R(EY(q)) - vy) + vb[2] * (R(EZ(q)) - vz)) * inv_rho;
+            put(q, (R(1) - om) * lam[q] + grho +
+                       (vb[0] * (R(EX(q)) - vx) + vb[1] * (R(EY(q)) - vy) + vb[2] * (R(EZ(q)) - vz)) * inv_rho);
           });
           // dJ/dK: via u (c0, c1) and via F (dk, fk)
           const R K2 = K * K, K32 = K * sK;
@@ -334,10 +354,20 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           if (A.grad_perm) A.grad_perm[cell] += (double)(du_dK + dF);
         }
       }
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        mu_out[(size_t)q * N + cell] = mu[q];
-      });
+    }
+  }
+  {
+    // Lanes that write no adjoint value (solid lanes of a launched chunk,
+    // inlet cells, whose adjoint populations no reader gathers) store zeros,
+    // so every 128-byte output line is written whole.  Dense launches do so
+    // only for 32-lane groups that hold a written cell.
+    const bool written = active && kind != POL_KIND_INLET;
+    const uint64_t any = __ballot(written);
+    const uint64_t group = 0xffffffffull << (threadIdx.x & 32);
+    if (inside && !written && (a.chunks != nullptr || (any & group) != 0)) {
+      R* __restrict__ mu_out = (R*)A.lam_out;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) *AD::at(mu_out, (AI)q * N + cell) = R(0);
     }
   }
   // block partials of dJ/du_in and J (FP64)
@@ -374,8 +404,18 @@ __global__ void __launch_bounds__(256) k_adjoint_final(const double* __restrict_
 hipError_t launch_adjoint(int storage, const AdjointArgs& A, const uint32_t* rflags, const Grid& g, hipStream_t s) {
   dim3 grid(g.launch), block(BLOCK);
   switch (storage) {
-    case POL_F32: hipLaunchKernelGGL((k_adjoint<float, float>), grid, block, 0, s, A, rflags); break;
-    case POL_F64: hipLaunchKernelGGL((k_adjoint<double, double>), grid, block, 0, s, A, rflags); break;
+    case POL_F32:
+      if (fits_addr32(Q, A.fw.N, sizeof(float)))
+        hipLaunchKernelGGL((k_adjoint<float, float, true>), grid, block, 0, s, A, rflags);
+      else
+        hipLaunchKernelGGL((k_adjoint<float, float, false>), grid, block, 0, s, A, rflags);
+      break;
+    case POL_F64:
+      if (fits_addr32(Q, A.fw.N, sizeof(double)))
+        hipLaunchKernelGGL((k_adjoint<double, double, true>), grid, block, 0, s, A, rflags);
+      else
+        hipLaunchKernelGGL((k_adjoint<double, double, false>), grid, block, 0, s, A, rflags);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -51,35 +51,37 @@ struct Phys {
   R inlet_rho, inlet_u[3], outlet_rho;
 };
 
-// 36 (feq_q - w_q) for density rho = 1 + rd (c = 36 w_q, exact)
-template <typename R>
-__device__ inline R deq_q(int ex, int ey, int ez, R c, R rd, R rho, R ux, R uy, R uz, R usq) {
-  const R eu = R(ex) * ux + R(ey) * uy + R(ez) * uz;
-  return c * (rd + rho * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
+// e . (x, y, z) for a lattice direction with components in {-1, 0, 1},
+// from additions only: x * 0 cannot be folded under IEEE rules (NaN, signed
+// zero), so a product with the component would keep every dead term alive.
+template <int c, typename R>
+__device__ __forceinline__ R signed_term(R v) {
+  if constexpr (c > 0) return v;
+  else return -v;
 }
-template <typename R>
-__device__ inline R feq_q(int ex, int ey, int ez, R w, R rho, R ux, R uy, R uz, R usq) {
-  const R eu = R(ex) * ux + R(ey) * uy + R(ez) * uz;
-  return w * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq);
+template <int ex, int ey, int ez, typename R>
+__device__ __forceinline__ R edot(R x, R y, R z) {
+  if constexpr (ex != 0 && ey != 0 && ez != 0) return signed_term<ex>(x) + signed_term<ey>(y) + signed_term<ez>(z);
+  else if constexpr (ex != 0 && ey != 0) return signed_term<ex>(x) + signed_term<ey>(y);
+  else if constexpr (ex != 0 && ez != 0) return signed_term<ex>(x) + signed_term<ez>(z);
+  else if constexpr (ey != 0 && ez != 0) return signed_term<ey>(y) + signed_term<ez>(z);
+  else if constexpr (ex != 0) return signed_term<ex>(x);
+  else if constexpr (ey != 0) return signed_term<ey>(y);
+  else if constexpr (ez != 0) return signed_term<ez>(z);
+  else return R(0);
+}
+// acc += c v for a component c in {-1, 0, 1}
+template <int c, typename R>
+__device__ __forceinline__ void eacc(R& acc, R v) {
+  if constexpr (c > 0) acc += v;
+  else if constexpr (c < 0) acc -= v;
 }
 
-// Velocity of a porous cell with Darcy-Forchheimer resistance, solved in
-// closed form: rho u = m + (F0 + F_DF(u))/2 with
-// F_DF = -rho (nu/K) u - rho (C_F/sqrt K) |u| u gives
-// u = v / (c0 + c1 |u|), v = (m + F0/2)/rho, c0 = 1 + nu/(2K), c1 = C_F/(2 sqrt K)
-// => u = 2 v / (c0 + sqrt(c0^2 + 4 c1 |v|)).
-template <typename R>
-__device__ inline void porous_velocity(R vx, R vy, R vz, R nu, R K, R CF, R& ux, R& uy, R& uz, R& dk, R& fk) {
-  const R sqK = sqrt(K);
-  const R c0 = R(1) + nu / (R(2) * K);
-  const R c1 = CF / (R(2) * sqK);
-  const R vmag = sqrt(vx * vx + vy * vy + vz * vz);
-  const R s = R(2) / (c0 + sqrt(c0 * c0 + R(4) * c1 * vmag));
-  ux = vx * s;
-  uy = vy * s;
-  uz = vz * s;
-  dk = nu / K;    // Darcy coefficient
-  fk = CF / sqK;  // Forchheimer coefficient
+// 36 (feq_q - w_q) for density rho = 1 + rd (c = 36 w_q, exact)
+template <typename R, int q>
+__device__ inline R deq_q(R c, R rd, R rho, R ux, R uy, R uz, R usq) {
+  const R eu = edot<EX(q), EY(q), EZ(q)>(ux, uy, uz);
+  return c * (rd + rho * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
 }
 
 // Moments of the collided state (for reductions / macro output).
@@ -107,7 +109,7 @@ __device__ inline void set_equilibrium(R (&d)[Q], R rd, R rho, R ux, R uy, R uz)
   static_for<0, 9>([&](auto pc) {
     constexpr int a = PAIR_A(decltype(pc)::value);
     constexpr int b = OPP(a);
-    const R eu = R(EX(a)) * ux + R(EY(a)) * uy + R(EZ(a)) * uz;
+    const R eu = edot<EX(a), EY(a), EZ(a)>(ux, uy, uz);
     const R sym = CW<R>(a) * (base + R(4.5) * rho * eu * eu);
     const R anti = R(3) * CW<R>(a) * rho * eu;
     d[a] = sym + anti;
@@ -213,7 +215,7 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
     static_for<0, 9>([&](auto pc) {
       constexpr int a = PAIR_A(decltype(pc)::value);
       constexpr int b = OPP(a);
-      const R eu = R(EX(a)) * ux + R(EY(a)) * uy + R(EZ(a)) * uz;
+      const R eu = edot<EX(a), EY(a), EZ(a)>(ux, uy, uz);
       const R n = (d[a] + d[b]) - R(2) * CW<R>(a) * (base + R(4.5) * rho * eu * eu);
       if constexpr (EX(a) != 0) pxx += n;
       if constexpr (EY(a) != 0) pyy += n;
@@ -241,14 +243,14 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
     constexpr int a = PAIR_A(decltype(pc)::value);
     constexpr int b = OPP(a);
     constexpr R c = CW<R>(a);
-    const R eu = R(EX(a)) * ux + R(EY(a)) * uy + R(EZ(a)) * uz;
+    const R eu = edot<EX(a), EY(a), EZ(a)>(ux, uy, uz);
     // omega multiplies the finished equilibrium terms: a rounded product such
     // as omega*3 would scale every cell's equilibrium momentum by the same
     // (1 + eps) and drift the flow systematically
     R sym = omega * (c * (base + R(4.5) * rho * eu * eu));
     R anti = omega * (R(3) * c * rho * eu);
     if constexpr (FULL) {
-      const R eF = R(EX(a)) * Fx + R(EY(a)) * Fy + R(EZ(a)) * Fz;
+      const R eF = edot<EX(a), EY(a), EZ(a)>(Fx, Fy, Fz);
       sym += cf * (c * (R(9) * eu * eF - R(3) * uF));
       anti += cf * (R(3) * c * eF);
     }

@@ -19,8 +19,9 @@ struct StepArgs {
   void* rho;          // real [N]
   void* u;            // real [3][N]
   double* partial;    // [nblk][POL_NRED]
-  const uint32_t* chunks;  // chunk list: item -> chunks[item / 32] + item % 32 (null: item = cell)
-  uint32_t n_items;        // number of work items (cells, or 32 x chunks)
+  const uint32_t* chunks;  // chunk list: item -> chunks[item >> cshift] + lane (null: item = cell)
+  uint32_t n_items;        // number of work items (cells, or chunk cells x chunks)
+  uint32_t cshift;         // log2 of the cells per chunk
   uint32_t N, nx, ny, nz, nxy;
   FastDiv dnx, dny;
   uint32_t per_xcd, nblk;

@@ -141,17 +141,51 @@ __device__ inline uint32_t logical_block(uint32_t per_xcd) {
   return (b % NXCD) * per_xcd + b / NXCD;
 }
 
+// Plane-major buffer addressing.  Addr<true> keeps the element index
+// q*N + i and its byte offset in 32 bits, so an access is one 64-bit SGPR base
+// plus one 32-bit VGPR offset (global saddr form) instead of a 64-bit VGPR
+// address pair; launchers select it when the whole buffer is below 4 GiB.
+template <bool A32>
+struct Addr;
+template <>
+struct Addr<true> {
+  using idx = uint32_t;
+  template <typename T>
+  __device__ static T* at(T* b, uint32_t i) {
+    return (T*)((char*)b + (uint32_t)(i * (uint32_t)sizeof(T)));
+  }
+  template <typename T>
+  __device__ static const T* at(const T* b, uint32_t i) {
+    return (const T*)((const char*)b + (uint32_t)(i * (uint32_t)sizeof(T)));
+  }
+};
+template <>
+struct Addr<false> {
+  using idx = size_t;
+  template <typename T>
+  __device__ static T* at(T* b, size_t i) { return b + i; }
+  template <typename T>
+  __device__ static const T* at(const T* b, size_t i) { return b + i; }
+};
+// true when `planes` planes of n elements of `bytes` each stay below 4 GiB
+inline bool fits_addr32(uint64_t planes, uint64_t n, uint64_t bytes) { return planes * n * bytes < (1ull << 32); }
+
 // Work item -> cell.  Dense launches map item = cell.  Chunk launches map
-// item to lane (item % POL_CHUNK) of row chunk chunks[item / POL_CHUNK]; a
-// lane past the row end (only when nx % POL_CHUNK != 0) is reported invalid.
-__device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ chunks,
-                                 const FastDiv& dnx, uint32_t nx, uint32_t& cell, uint32_t& yz, uint32_t& x) {
+// item to lane (item & (2^cshift - 1)) of row chunk chunks[item >> cshift]; a
+// lane past the row end, or a cell index past the grid, is reported invalid,
+// so no list content leads a kernel outside the buffers.
+__device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ chunks, uint32_t cshift,
+                                 uint32_t N, const FastDiv& dnx, uint32_t nx, uint32_t& cell, uint32_t& yz,
+                                 uint32_t& x) {
   bool ok = item < n_items;
+  const uint32_t lane = item & ((1u << cshift) - 1u);
   uint32_t c = 0;
-  if (ok) c = chunks ? chunks[item / POL_CHUNK] + (item % POL_CHUNK) : item;
+  if (ok) c = chunks ? chunks[item >> cshift] + lane : item;
+  ok = ok && c < N;
+  if (!ok) c = 0;
   yz = fdiv(c, dnx);
   x = c - yz * nx;
-  if (chunks && x < (item % POL_CHUNK)) ok = false;  // wrapped into the next row
+  if (chunks && x < lane) ok = false;  // wrapped into the next row
   cell = ok ? c : 0u;
   if (!ok) {
     yz = 0;

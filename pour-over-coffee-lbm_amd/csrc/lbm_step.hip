@@ -20,14 +20,16 @@
 
 namespace pol {
 
-template <typename S, bool FLAGS, bool FULL, bool STORE>
+template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32>
 __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   using C = Codec<S>;
   using R = typename C::real;
+  using AD = Addr<A32>;
+  using AI = typename AD::idx;
   const uint32_t lblk = logical_block(a.per_xcd);
   if (lblk >= a.nblk) return;  // whole padding block exits (uniform)
   uint32_t cell, yz, x;
-  const bool inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.dnx, a.nx, cell, yz, x);
+  const bool inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
   const uint32_t z = fdiv(yz, a.dny);
   const uint32_t y = yz - z * a.ny;
 
@@ -73,13 +75,13 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
         constexpr int q = decltype(qc)::value;
         const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
                         (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
-        size_t src = (size_t)q * N + (uint32_t)((int)cell + off);
+        AI src = (AI)q * N + (uint32_t)((int)cell + off);
         if constexpr (FLAGS && q != 0) {
-          if (fl & (1u << q)) src = (size_t)OPP(q) * N + cell;
+          if (fl & (1u << q)) src = (AI)OPP(q) * N + cell;
         }
         // bounce-back reads D*_{opp q}(x); w_{opp q} = w_q, so the stored
         // deviation of direction opp(q) is the deviation of link q.
-        f[q] = C::load(fin + src);
+        f[q] = C::load(AD::at(fin, src));
       });
     }
     R fx = R(0), fy = R(0), fz = R(0), K = P.K;
@@ -130,7 +132,7 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     if (store) {
       static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        C::store(fout + (size_t)q * N + cell, f[q]);
+        C::store(AD::at(fout, (AI)q * N + cell), f[q]);
       });
     }
   }
@@ -210,8 +212,28 @@ __global__ void __launch_bounds__(256) k_init_equilibrium(S* __restrict__ f, con
   const R usq = ux * ux + uy * uy + uz * uz;
   static_for<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    C::store(f + (size_t)q * N + i, deq_q<R>(EX(q), EY(q), EZ(q), CW<R>(q), r - R(1), r, ux, uy, uz, usq));
+    C::store(f + (size_t)q * N + i, deq_q<R, q>(CW<R>(q), r - R(1), r, ux, uy, uz, usq));
   });
+}
+
+template <typename S, bool A32>
+static void launch_sc(const StepArgs& a, bool store_f, bool flags, bool full, dim3 grid, hipStream_t s) {
+  dim3 block(BLOCK);
+  if (store_f) {
+    if (flags)
+      hipLaunchKernelGGL((k_stream_collide<S, true, true, true, A32>), grid, block, 0, s, a);
+    else if (full || !A32)
+      hipLaunchKernelGGL((k_stream_collide<S, false, true, true, A32>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_stream_collide<S, false, false, true, true>), grid, block, 0, s, a);
+  } else {
+    if (flags)
+      hipLaunchKernelGGL((k_stream_collide<S, true, true, false, A32>), grid, block, 0, s, a);
+    else if (full || !A32)
+      hipLaunchKernelGGL((k_stream_collide<S, false, true, false, A32>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_stream_collide<S, false, false, false, true>), grid, block, 0, s, a);
+  }
 }
 
 template <typename S>
@@ -219,22 +241,12 @@ static hipError_t dispatch_sc(const StepArgs& a, bool store_f, const Grid& g, hi
   const bool flags = a.flags != nullptr;
   const bool full =
       flags || (a.options & (POL_LES | POL_FORCE_FIELD | POL_PERM_FIELD | POL_GRAVITY | POL_PARTICLE_FIELD));
-  dim3 grid(g.launch), block(BLOCK);
-  if (store_f) {
-    if (flags)
-      hipLaunchKernelGGL((k_stream_collide<S, true, true, true>), grid, block, 0, s, a);
-    else if (full)
-      hipLaunchKernelGGL((k_stream_collide<S, false, true, true>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((k_stream_collide<S, false, false, true>), grid, block, 0, s, a);
-  } else {
-    if (flags)
-      hipLaunchKernelGGL((k_stream_collide<S, true, true, false>), grid, block, 0, s, a);
-    else if (full)
-      hipLaunchKernelGGL((k_stream_collide<S, false, true, false>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((k_stream_collide<S, false, false, false>), grid, block, 0, s, a);
-  }
+  // 32-bit offsets when a population buffer is below 4 GiB; larger grids take
+  // the general (FULL) kernel with 64-bit addressing
+  if (fits_addr32(Q, a.N, sizeof(S)))
+    launch_sc<S, true>(a, store_f, flags, full, dim3(g.launch), s);
+  else
+    launch_sc<S, false>(a, store_f, flags, full, dim3(g.launch), s);
   return hipGetLastError();
 }
 

@@ -15,6 +15,11 @@ import torch
 from . import _lib as L
 
 POL_CHUNK = L.POL_CHUNK
+WRITE_GRANULE = 128  # bytes: HBM writes narrower than this line cost a read-modify-write
+# cells per launch chunk by storage (measured at 224^3: FP32 16 / 32 / 64-cell
+# chunks 0.174 / 0.151 / 0.159 ms, 16-bit 16 / 32 / 64 cells 0.114 / 0.103 /
+# 0.110 ms)
+CHUNK_CELLS = {"f32": 32, "f16": 32, "bf16": 32, "f64": 16}
 _STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 
 
@@ -104,7 +109,7 @@ def _as_int32_view(a: np.ndarray) -> torch.Tensor:
 class HipContext:
     """One pol_ctx with its device buffers."""
 
-    def __init__(self, cfg: SolverConfig, flags: Optional[np.ndarray] = None):
+    def __init__(self, cfg: SolverConfig, flags: Optional[np.ndarray] = None, chunk_cells: Optional[int] = None):
         if not torch.cuda.is_available():
             raise RuntimeError("HipContext needs a visible HIP device (torch.cuda.is_available() is False)")
         self.lib = L.load()
@@ -119,6 +124,7 @@ class HipContext:
         self.rho = torch.ones(shape, dtype=rd, device=self.device)
         self.u = torch.zeros((3,) + shape, dtype=rd, device=self.device)
         self.flags = _as_int32_view(flags).to(self.device) if flags is not None else None
+        self.chunk_cells = int(chunk_cells or CHUNK_CELLS[cfg.precision])
         self.chunks = self._chunk_list(flags)
         self.force = torch.zeros((3,) + shape, dtype=rd, device=self.device) if cfg.force_field else None
         self.perm = torch.full(shape, cfg.darcy_K, dtype=rd, device=self.device) if cfg.perm_field else None
@@ -142,34 +148,36 @@ class HipContext:
         if self.chunks is not None:
             b.active_chunks = self.chunks.data_ptr()
             b.n_chunks = self.chunks.numel()
+            b.chunk_cells = self.chunk_cells
         b.particle_drag = self.pdrag.data_ptr() if self.pdrag is not None else None
         L.check(self.lib.pol_bind(self._ctx, C.byref(b)), self._ctx, "pol_bind")
 
     def _chunk_list(self, flags: Optional[np.ndarray]) -> Optional[torch.Tensor]:
-        """Launch list: first cells of the POL_CHUNK-cell row chunks holding any
-        non-solid cell (None when every chunk does or there are no flags).
+        """Launch list: first cells of the chunk_cells-cell row chunks holding
+        any non-solid cell (None when every chunk does or there are no flags).
         Chunks of solid cells get no threads; solid lanes of a listed chunk
-        store zero deviations so output lines are written whole.  With 16-bit
-        storage a 128-byte line spans two chunks, so both chunks of a line are
-        listed when either holds fluid (rows of a multiple of POL_CHUNK cells)."""
+        store zero deviations so written granules are whole.  When a chunk is
+        narrower than the write granule, the chunks sharing a granule are listed
+        together (rows of a multiple of the granule's cells)."""
         if flags is None:
             return None
         cfg = self.config
+        ch = self.chunk_cells
         kind = (np.asarray(flags, dtype=np.uint32) >> L.POL_KIND_SHIFT) & L.POL_KIND_MASK
         open_cell = kind != L.POL_KIND_SOLID
-        nch = -(-cfg.nx // POL_CHUNK)
-        pad = nch * POL_CHUNK - cfg.nx
+        nch = -(-cfg.nx // ch)
+        pad = nch * ch - cfg.nx
         if pad:
             open_cell = np.concatenate([open_cell, np.zeros(open_cell.shape[:2] + (pad,), bool)], axis=2)
-        live = open_cell.reshape(cfg.nz, cfg.ny, nch, POL_CHUNK).any(axis=3)
-        per_line = 128 // (cfg.bytes_per_population * POL_CHUNK)
-        if not pad and per_line > 1 and live.size % per_line == 0:
-            grp = live.reshape(-1, per_line).any(axis=1, keepdims=True)
-            live = np.broadcast_to(grp, (grp.shape[0], per_line)).reshape(live.shape)
+        live = open_cell.reshape(cfg.nz, cfg.ny, nch, ch).any(axis=3)
+        per_granule = WRITE_GRANULE // (cfg.bytes_per_population * ch)
+        if not pad and per_granule > 1 and (cfg.nx // ch) % per_granule == 0:
+            grp = live.reshape(-1, per_granule).any(axis=1, keepdims=True)
+            live = np.broadcast_to(grp, (grp.shape[0], per_granule)).reshape(live.shape)
         zz, yy, cc = np.nonzero(live)
         if zz.size == 0 or zz.size == live.size:
             return None
-        base = ((zz.astype(np.int64) * cfg.ny + yy) * cfg.nx + cc * POL_CHUNK).astype(np.uint32)
+        base = ((zz.astype(np.int64) * cfg.ny + yy) * cfg.nx + cc * ch).astype(np.uint32)
         return _as_int32_view(base).to(self.device)
 
     def close(self) -> None:

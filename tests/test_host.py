@@ -17,7 +17,7 @@ from pourover_amd import backends as B
 from pourover_amd import fields as F
 from pourover_amd import geometry as G
 from pourover_amd import workloads as WL
-from pourover_amd.engine import POL_CHUNK, HipContext, SolverConfig
+from pourover_amd.engine import CHUNK_CELLS, WRITE_GRANULE, HipContext, SolverConfig
 from pourover_amd.solver import HipLBMSolver, LBMSolverProtocol, ParticleCoupledLBMProtocol
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -126,36 +126,41 @@ def test_permeability_field_range():
 
 # ---- launch chunk list ------------------------------------------------------------
 class _Cfg:
-    def __init__(self, cfg):
+    def __init__(self, cfg, chunk_cells=None):
         self.config = cfg
         self.device = "cpu"
+        self.chunk_cells = chunk_cells or CHUNK_CELLS[cfg.precision]
 
 
-def _chunks(cfg, flags):
-    c = HipContext._chunk_list(_Cfg(cfg), flags)
+def _chunks(cfg, flags, chunk_cells=None):
+    c = HipContext._chunk_list(_Cfg(cfg, chunk_cells), flags)
     return None if c is None else c.numpy().view(np.uint32)
 
 
-@pytest.mark.parametrize("prec", ["f32", "f16", "f64"])
-def test_chunk_list_covers_every_open_cell(prec):
+@pytest.mark.parametrize("prec,ch", [("f32", None), ("f16", None), ("f64", None), ("f16", 16), ("f32", 8),
+                                     ("f32", 32)])
+def test_chunk_list_covers_every_open_cell(prec, ch):
     wl = WL.v60_config(64, prec)
     flags = wl.geometry.flags
-    base = _chunks(wl.config, flags)
+    ch = ch or CHUNK_CELLS[prec]
+    base = _chunks(wl.config, flags, ch)
     assert base is not None and (np.diff(base.astype(np.int64)) > 0).all()
-    assert (base % POL_CHUNK == 0).all()
+    assert (base % ch == 0).all()
     covered = np.zeros(flags.size, bool)
     for c in base:
-        covered[c:c + POL_CHUNK] = True
+        covered[c:c + ch] = True
     open_cell = (wl.geometry.kind != L.POL_KIND_SOLID).ravel()
     assert covered[open_cell].all()
-    if prec in ("f16",):
-        # both 32-cell halves of every 128-byte line are listed together
-        line = base // (128 // 2)
-        _, cnt = np.unique(line, return_counts=True)
-        assert (cnt == 2).all()
+    bpp = wl.config.bytes_per_population
+    per = WRITE_GRANULE // (bpp * ch)
+    if per > 1:
+        # the chunks sharing a 64-byte granule are listed together
+        _, cnt = np.unique(base // (ch * per), return_counts=True)
+        assert (cnt == per).all()
     else:
-        # only chunks that hold fluid
-        assert all(open_cell[c:c + POL_CHUNK].any() for c in base)
+        # only chunks that hold fluid, each a whole number of granules
+        assert (ch * bpp) % WRITE_GRANULE == 0
+        assert all(open_cell[c:c + ch].any() for c in base)
 
 
 def test_chunk_list_absent_for_open_box():
