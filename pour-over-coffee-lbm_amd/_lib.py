@@ -123,7 +123,7 @@ def load() -> C.CDLL:
     global _lib
     if _lib is not None:
         return _lib
-    path = Path(os.environ.get("POUROVER_HIP_LIB", LIB_PATH))
+    path = Path(os.environ.get("POUROVER_HIP_LIB") or LIB_PATH)
     if not path.exists():
         raise ImportError(
             "libpourover_hip.so not found at %s: build it with __graft_entry__.build() "

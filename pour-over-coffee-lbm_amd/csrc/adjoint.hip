@@ -28,23 +28,46 @@ namespace pol {
 //   bit p (0..18): the normal reader y + e_p of population p exists and gathers
 //   bit 31       : y lies on a non-periodic face where clamped readers may exist
 
+// Weighted moments of the adjoint populations: every sum the collision VJP
+// needs is a polynomial in (u, F) of these ten numbers,
+//   M0 = sum_p w_p lam_p,  M1 = sum_p w_p lam_p e_p,  M2 = sum_p w_p lam_p e_p e_p,
+// accumulated with additions only (e components are -1, 0, 1).
 template <typename R>
-__device__ inline void eq_grad_u(R rho, R ux, R uy, R uz, const R (&lam)[Q], R& ax, R& ay, R& az) {
-  // sum_p lam_p d feq_p(rho,u)/du = rho sum_p lam_p w_p (3 e_p + 9 (e_p.u) e_p - 3 u)
-  R sx = 0, sy = 0, sz = 0, s0 = 0;
+struct LamMoments {
+  R m0, m1x, m1y, m1z, xx, yy, zz, xy, xz, yz;
+  __device__ void m2dot(R x, R y, R z, R& ox, R& oy, R& oz) const {
+    ox = xx * x + xy * y + xz * z;
+    oy = xy * x + yy * y + yz * z;
+    oz = xz * x + yz * y + zz * z;
+  }
+};
+template <typename R>
+__device__ inline LamMoments<R> lam_moments(const R (&lam)[Q]) {
+  LamMoments<R> M{};
   static_for<0, Q>([&](auto qc) {
     constexpr int p = decltype(qc)::value;
     const R lw = lam[p] * W<R>(p);
-    const R eu = edot<EX(p), EY(p), EZ(p)>(ux, uy, uz);
-    const R c = R(3) * lw + R(9) * lw * eu;
-    eacc<EX(p)>(sx, c);
-    eacc<EY(p)>(sy, c);
-    eacc<EZ(p)>(sz, c);
-    s0 += lw;
+    M.m0 += lw;
+    eacc<EX(p)>(M.m1x, lw);
+    eacc<EY(p)>(M.m1y, lw);
+    eacc<EZ(p)>(M.m1z, lw);
+    eacc<EX(p) * EX(p)>(M.xx, lw);
+    eacc<EY(p) * EY(p)>(M.yy, lw);
+    eacc<EZ(p) * EZ(p)>(M.zz, lw);
+    eacc<EX(p) * EY(p)>(M.xy, lw);
+    eacc<EX(p) * EZ(p)>(M.xz, lw);
+    eacc<EY(p) * EZ(p)>(M.yz, lw);
   });
-  ax = rho * (sx - R(3) * ux * s0);
-  ay = rho * (sy - R(3) * uy * s0);
-  az = rho * (sz - R(3) * uz * s0);
+  return M;
+}
+// sum_p lam_p d feq_p(rho, u)/du = rho (3 M1 + 9 M2 u - 3 u M0)
+template <typename R>
+__device__ inline void eq_grad_u(const LamMoments<R>& M, R rho, R ux, R uy, R uz, R& ax, R& ay, R& az) {
+  R tx, ty, tz;
+  M.m2dot(ux, uy, uz, tx, ty, tz);
+  ax = rho * (R(3) * M.m1x + R(9) * tx - R(3) * ux * M.m0);
+  ay = rho * (R(3) * M.m1y + R(9) * ty - R(3) * uy * M.m0);
+  az = rho * (R(3) * M.m1z + R(9) * tz - R(3) * uz * M.m0);
 }
 
 struct AdjGeom {
@@ -79,23 +102,36 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   const int ozm = z == 0 ? (pz ? (nz - 1) * nxy : 0) : -nxy;
   const int ozp = z == a.nz - 1 ? (pz ? -(nz - 1) * nxy : 0) : nxy;
 
+  // Every lane's output has the form
+  //   mu_q = keep lam_q + C + e_q.h + L1 e_q.P.e_q
+  // (fluid, porous and outlet cells set the scalars; solid lanes of a launched
+  // chunk and inlet cells, whose adjoint no reader gathers, keep them zero),
+  // so one shared loop writes the 19 values for every lane kind.
+  R lam[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) lam[q] = R(0);
+  R keep = 0, C = 0, hx = 0, hy = 0, hz = 0, L1 = 0;
+  R Pxx = 0, Pyy = 0, Pzz = 0, Pxy2 = 0, Pxz2 = 0, Pyz2 = 0;
   if (active) {
     const R* __restrict__ mu_in = (const R*)A.lam_in;
-    R* __restrict__ mu_out = (R*)A.lam_out;
     // ---- 1. lambda_p = dJ/df^{n+1}_p(x), reverse gather of mu^{n+1} ----
-    R lam[Q];
     const uint32_t rf = rflags[cell];
     static_for<0, Q>([&](auto qc) {
       constexpr int p = decltype(qc)::value;
       // normal reader at x + e_p
       const int off = (EX(p) > 0 ? oxp : (EX(p) < 0 ? oxm : 0)) + (EY(p) > 0 ? oyp : (EY(p) < 0 ? oym : 0)) +
                       (EZ(p) > 0 ? ozp : (EZ(p) < 0 ? ozm : 0));
-      R v = R(0);
-      if (rf & (1u << p)) v = *AD::at(mu_in, (AI)p * N + (uint32_t)((int)cell + off));
-      if constexpr (p != 0) {
-        if (kind != POL_KIND_INLET && (fl & (1u << OPP(p)))) v += *AD::at(mu_in, (AI)OPP(p) * N + cell);
-      }
-      lam[p] = v;
+      // The reader of f_p(x) is either x + e_p (normal link, rev bit p) or x
+      // itself through its bounce link opp(p) -- never both: the bounce means
+      // x + e_p is a wall or outside the box, which has no normal reader.  One
+      // unconditional load from a select of the two addresses (own cell when
+      // neither applies, then zeroed) keeps the gather free of branches.
+      const bool nb = rf & (1u << p);
+      bool bb = false;
+      if constexpr (p != 0) bb = kind != POL_KIND_INLET && (fl & (1u << OPP(p)));
+      const AI src = nb ? (AI)p * N + (uint32_t)((int)cell + off) : (AI)OPP(p) * N + cell;
+      const R v = *AD::at(mu_in, src);
+      lam[p] = (nb || bb) ? v : R(0);
     });
     if (rf & POL_RFLAG_CLAMP) {
       // readers whose out-of-domain source was clamped onto this cell
@@ -117,7 +153,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
             } else {
               r[d] = c[d] + e[d];
               if (r[d] < 0 || r[d] >= n3[d]) {
-                if (per[d]) r[d] = (r[d] + n3[d]) % n3[d];
+                if (per[d]) r[d] += r[d] < 0 ? n3[d] : -n3[d];  // one period away at most
                 else ok = false;
               }
             }
@@ -132,16 +168,18 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       });
     }
 
+    const LamMoments<R> M = lam_moments<R>(lam);
+
     // ---- 2. recompute the forward cell state ----
     const R G0 = (R)a.G[0], G1 = (R)a.G[1], G2 = (R)a.G[2];
     if (kind == POL_KIND_INLET) {
       const R ri = (R)a.inlet_rho, ux = (R)a.inlet_u[0], uy = (R)a.inlet_u[1], uz = (R)a.inlet_u[2];
       R gx, gy, gz;
-      eq_grad_u<R>(ri, ux, uy, uz, lam, gx, gy, gz);
+      eq_grad_u<R>(M, ri, ux, uy, uz, gx, gy, gz);
       part[0] = (double)gx;
       part[1] = (double)gy;
       part[2] = (double)gz;
-      // nothing gathered: mu at inlet cells is never read
+      // nothing gathered: mu at inlet cells is never read (left zero)
     } else {
       const S* __restrict__ fin = (const S*)a.fin;
       // Stream the gathered cell's stored deviations D_q = 36 (g_q - w_q)
@@ -193,17 +231,16 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       }
       const R vx = (mx + R(0.5) * F0x) * inv_rho, vy = (my + R(0.5) * F0y) * inv_rho,
               vz = (mz + R(0.5) * F0z) * inv_rho;
-      auto put = [&](int q, R v) { *AD::at(mu_out, (AI)q * N + cell) = v; };
       if (kind == POL_KIND_OUTLET) {
         const R ro = (R)a.outlet_rho;
         R ax, ay, az;
-        eq_grad_u<R>(ro, vx, vy, vz, lam, ax, ay, az);
-        static_for<0, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          R m = (ax * (R(EX(q)) - vx) + ay * (R(EY(q)) - vy) + az * (R(EZ(q)) - vz)) * inv_rho;
-          if (A.objective) m -= R(EZ(q));
-          put(q, m);
-        });
+        eq_grad_u<R>(M, ro, vx, vy, vz, ax, ay, az);
+        // mu_q = a.(e_q - v)/rho - [objective] e_qz
+        hx = ax * inv_rho;
+        hy = ay * inv_rho;
+        hz = az * inv_rho;
+        C = -(hx * vx + hy * vy + hz * vz);
+        if (A.objective) hz -= R(1);
         if (A.objective) part[3] = -(double)(mz + R(0.5) * F0z);
       } else {
         const bool porous = fl & POL_FLAG_POROUS;
@@ -257,65 +294,63 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         }
         const R om = R(1) / tau;
         const R cf = R(1) - R(0.5) * om;
-        // ---- 3. adjoint of the collision ----
-        // A_rho = sum lam fe / rho ; A_u = sum lam dfe/du ; B_u = sum lam dS/du ;
-        // C_F = sum lam dS/dF ; dJ/domega
-        R Arho = 0, Sw = 0, domega = 0;
-        R lwe[3] = {0, 0, 0}, lweu[3] = {0, 0, 0}, lweF[3] = {0, 0, 0};
+        // ---- 3. adjoint of the collision, from the lambda moments ----
+        //   sum lam feq / rho  = (M0 + rd M0 + rho (3 M1.u + 4.5 u.M2.u - 1.5 u.u M0)) / rho
+        //   sum lam dfeq/du    = rho (3 M1 + 9 M2 u - 3 u M0)                  (Au)
+        //   sum lam dS/du      = cf (9 M2 F - 3 F M0)                          (Bu)
+        //   sum lam dS/dF      = cf (3 M1 - 3 u M0 + 9 M2 u)                   (CF3)
+        //   sum lam df*/domega = sum lam (feq - g) - (3 M1.F - 3 u.F M0 + 9 u.M2.F) / 2
         const R uF = ux * Fx + uy * Fy + uz * Fz;
-        static_for<0, Q>([&](auto qc) {
-          constexpr int p = decltype(qc)::value;
-          const R lw = lam[p] * W<R>(p);
-          const R eu = edot<EX(p), EY(p), EZ(p)>(ux, uy, uz);
-          const R eF = edot<EX(p), EY(p), EZ(p)>(Fx, Fy, Fz);
-          // feq_p - w_p = w_p (rd + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u))
-          const R fdev = W<R>(p) * (rd + rho * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * usq));
-          Arho += lam[p] * fdev;
-          Sw += lw;
-          const R lweu_p = lw * eu, lweF_p = lw * eF;
-          eacc<EX(p)>(lwe[0], lw);
-          eacc<EY(p)>(lwe[1], lw);
-          eacc<EZ(p)>(lwe[2], lw);
-          eacc<EX(p)>(lweu[0], lweu_p);
-          eacc<EY(p)>(lweu[1], lweu_p);
-          eacc<EZ(p)>(lweu[2], lweu_p);
-          eacc<EX(p)>(lweF[0], lweF_p);
-          eacc<EY(p)>(lweF[1], lweF_p);
-          eacc<EZ(p)>(lweF[2], lweF_p);
-          // d f*_p / d omega = fe_p - g_p - S_p / (2 cf)
-          domega += lam[p] * fdev - R(0.5) * lw * (R(3) * (eF - uF) + R(9) * eu * eF);
-        });
-        domega -= lamD * i36;  // the -g_p part of fe_p - g_p, less the w_p in both
-        Arho = (Arho + Sw) * inv_rho;  // sum lam feq / rho with feq = w + fdev
-        const R u3[3] = {ux, uy, uz}, F3[3] = {Fx, Fy, Fz};
+        R M2u[3], M2F[3];
+        M.m2dot(ux, uy, uz, M2u[0], M2u[1], M2u[2]);
+        M.m2dot(Fx, Fy, Fz, M2F[0], M2F[1], M2F[2]);
+        const R M1u = M.m1x * ux + M.m1y * uy + M.m1z * uz;
+        const R M1F = M.m1x * Fx + M.m1y * Fy + M.m1z * Fz;
+        const R uM2u = ux * M2u[0] + uy * M2u[1] + uz * M2u[2];
+        const R uM2F = ux * M2F[0] + uy * M2F[1] + uz * M2F[2];
+        const R lam_fdev = rd * M.m0 + rho * (R(3) * M1u + R(4.5) * uM2u - R(1.5) * usq * M.m0);
+        // the -g part: sum lam (w + D/36), less the w_p common to feq and g
+        const R domega = lam_fdev - lamD * i36 - R(0.5) * (R(3) * M1F - R(3) * uF * M.m0 + R(9) * uM2F);
+        const R Arho = (lam_fdev + M.m0) * inv_rho;
+        const R u3[3] = {ux, uy, uz}, F3[3] = {Fx, Fy, Fz}, M1[3] = {M.m1x, M.m1y, M.m1z};
         R Au[3], Bu[3], CF3[3];
         for (int d = 0; d < 3; ++d) {
-          Au[d] = rho * (R(3) * lwe[d] + R(9) * lweu[d] - R(3) * u3[d] * Sw);
-          Bu[d] = cf * (R(-3) * F3[d] * Sw + R(9) * lweF[d]);
+          Au[d] = rho * (R(3) * M1[d] + R(9) * M2u[d] - R(3) * u3[d] * M.m0);
+          Bu[d] = cf * (R(-3) * F3[d] * M.m0 + R(9) * M2F[d]);
           // dS/dF = cf w (3 (e - u) + 9 (e.u) e)
-          CF3[d] = cf * (R(3) * lwe[d] - R(3) * u3[d] * Sw + R(9) * lweu[d]);
+          CF3[d] = cf * (R(3) * M1[d] - R(3) * u3[d] * M.m0 + R(9) * M2u[d]);
         }
+        keep = R(1) - om;
         if (!porous) {
-          R gu[3];
-          for (int d = 0; d < 3; ++d) gu[d] = om * Au[d] + Bu[d];
-          const R base = om * Arho;
-          R Lam = 0, trP3 = 0, iQn = 0;
+          // mu_q = (1-om) lam_q + om Arho + gu.(e_q - u)/rho
+          //        + Lam ((w.P.w - trP/3)/Q - Q/rho),  w = e_q - u   (LES)
+          // with w.P.w = e.P.e - 2 e.(P u) + u.P.u, so per direction only
+          // additions remain: mu_q = keep lam_q + C + e.h + L1 e.P.e
+          const R gx = (om * Au[0] + Bu[0]) * inv_rho, gy = (om * Au[1] + Bu[1]) * inv_rho,
+                  gz = (om * Au[2] + Bu[2]) * inv_rho;
+          C = om * Arho - (gx * ux + gy * uy + gz * uz);
+          hx = gx;
+          hy = gy;
+          hz = gz;
           if (les_active) {
-            Lam = domega * (-om * om) * (R)a.les_c / (R(4) * sroot * rho);
-            trP3 = (pxx + pyy + pzz) * R(1.0 / 3.0);
-            iQn = R(1) / Qn;
+            const R Lam = domega * (-om * om) * (R)a.les_c / (R(4) * sroot * rho);
+            const R iQn = R(1) / Qn;
+            const R trP3 = (pxx + pyy + pzz) * R(1.0 / 3.0);
+            const R Pux = pxx * ux + pxy * uy + pxz * uz, Puy = pxy * ux + pyy * uy + pyz * uz,
+                    Puz = pxz * ux + pyz * uy + pzz * uz;
+            const R uPu = ux * Pux + uy * Puy + uz * Puz;
+            L1 = Lam * iQn;
+            C += Lam * ((uPu - trP3) * iQn - Qn * inv_rho);
+            hx -= R(2) * L1 * Pux;
+            hy -= R(2) * L1 * Puy;
+            hz -= R(2) * L1 * Puz;
+            Pxx = pxx;
+            Pyy = pyy;
+            Pzz = pzz;
+            Pxy2 = R(2) * pxy;
+            Pxz2 = R(2) * pxz;
+            Pyz2 = R(2) * pyz;
           }
-          static_for<0, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const R wx = R(EX(q)) - ux, wy = R(EY(q)) - uy, wz = R(EZ(q)) - uz;
-            R m = (R(1) - om) * lam[q] + base + (gu[0] * wx + gu[1] * wy + gu[2] * wz) * inv_rho;
-            if (les_active) {
-              const R wPw = wx * wx * pxx + wy * wy * pyy + wz * wz * pzz +
-                            R(2) * (wx * wy * pxy + wx * wz * pxz + wy * wz * pyz);
-              m += Lam * ((wPw - trP3) * iQn - Qn * inv_rho);
-            }
-            put(q, m);
-          });
         } else {
           // F = F0 - rho (dk + fk |u|) u ; u = v s(|v|)
           const R cdot = CF3[0] * ux + CF3[1] * uy + CF3[2] * uz;
@@ -337,11 +372,11 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
             vb[1] += t * vy;
             vb[2] += t * vz;
           }
-          static_for<0, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            put(q, (R(1) - om) * lam[q] + grho +
-                       (vb[0] * (R(EX(q)) - vx) + vb[1] * (R(EY(q)) - vy) + vb[2] * (R(EZ(q)) - vz)) * inv_rho);
-          });
+          // mu_q = (1-om) lam_q + grho + vb.(e_q - v)/rho
+          hx = vb[0] * inv_rho;
+          hy = vb[1] * inv_rho;
+          hz = vb[2] * inv_rho;
+          C = grho - (hx * vx + hy * vy + hz * vz);
           // dJ/dK: via u (c0, c1) and via F (dk, fk)
           const R K2 = K * K, K32 = K * sK;
           const R dc0 = -nu0 / (R(2) * K2);
@@ -357,17 +392,24 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
     }
   }
   {
-    // Lanes that write no adjoint value (solid lanes of a launched chunk,
-    // inlet cells, whose adjoint populations no reader gathers) store zeros,
-    // so every 128-byte output line is written whole.  Dense launches do so
-    // only for 32-lane groups that hold a written cell.
-    const bool written = active && kind != POL_KIND_INLET;
-    const uint64_t any = __ballot(written);
+    // One store loop for every lane kind.  Solid lanes of a launched chunk
+    // store too (zeros), so every 128-byte output line is written whole;
+    // dense launches do so only for 32-lane groups that hold a non-solid cell.
+    const uint64_t any = __ballot(active);
     const uint64_t group = 0xffffffffull << (threadIdx.x & 32);
-    if (inside && !written && (a.chunks != nullptr || (any & group) != 0)) {
+    if (inside && (active || a.chunks != nullptr || (any & group) != 0)) {
       R* __restrict__ mu_out = (R*)A.lam_out;
-#pragma unroll
-      for (int q = 0; q < Q; ++q) *AD::at(mu_out, (AI)q * N + cell) = R(0);
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        R ePe = R(0);
+        eacc<EX(q) * EX(q)>(ePe, Pxx);
+        eacc<EY(q) * EY(q)>(ePe, Pyy);
+        eacc<EZ(q) * EZ(q)>(ePe, Pzz);
+        eacc<EX(q) * EY(q)>(ePe, Pxy2);
+        eacc<EX(q) * EZ(q)>(ePe, Pxz2);
+        eacc<EY(q) * EZ(q)>(ePe, Pyz2);
+        *AD::at(mu_out, (AI)q * N + cell) = keep * lam[q] + C + edot<EX(q), EY(q), EZ(q)>(hx, hy, hz) + L1 * ePe;
+      });
     }
   }
   // block partials of dJ/du_in and J (FP64)

@@ -22,6 +22,7 @@ struct StepArgs {
   const uint32_t* chunks;  // chunk list: item -> chunks[item >> cshift] + lane (null: item = cell)
   uint32_t n_items;        // number of work items (cells, or chunk cells x chunks)
   uint32_t cshift;         // log2 of the cells per chunk
+  uint32_t stream_store;   // select the non-temporal-store kernels (output far above the last-level cache)
   uint32_t N, nx, ny, nz, nxy;
   FastDiv dnx, dny;
   uint32_t per_xcd, nblk;

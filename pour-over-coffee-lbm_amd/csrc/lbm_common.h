@@ -58,6 +58,7 @@ struct Codec<float> {
   using real = float;
   __device__ static inline float load(const float* p) { return *p; }
   __device__ static inline void store(float* p, float v) { *p = v; }
+  __device__ static inline void store_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
 };
 
 template <>
@@ -65,6 +66,7 @@ struct Codec<double> {
   using real = double;
   __device__ static inline double load(const double* p) { return *p; }
   __device__ static inline void store(double* p, double v) { *p = v; }
+  __device__ static inline void store_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
 };
 
 template <>
@@ -72,6 +74,7 @@ struct Codec<_Float16> {
   using real = float;
   __device__ static inline float load(const _Float16* p) { return (float)(*p); }
   __device__ static inline void store(_Float16* p, float v) { *p = (_Float16)v; }
+  __device__ static inline void store_nt(_Float16* p, float v) { __builtin_nontemporal_store((_Float16)v, p); }
 };
 
 __device__ inline float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
@@ -88,6 +91,7 @@ struct Codec<bf16_t> {
   using real = float;
   __device__ static inline float load(const bf16_t* p) { return bf16_to_f32(p->bits); }
   __device__ static inline void store(bf16_t* p, float v) { p->bits = f32_to_bf16(v); }
+  __device__ static inline void store_nt(bf16_t* p, float v) { __builtin_nontemporal_store(f32_to_bf16(v), &p->bits); }
 };
 
 // ---------------------------------------------------------------------------

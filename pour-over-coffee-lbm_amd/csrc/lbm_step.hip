@@ -20,7 +20,7 @@
 
 namespace pol {
 
-template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32>
+template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32, bool NT = false>
 __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   using C = Codec<S>;
   using R = typename C::real;
@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     if (store) {
       static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        C::store(AD::at(fout, (AI)q * N + cell), f[q]);
+        if constexpr (NT) C::store_nt(AD::at(fout, (AI)q * N + cell), f[q]);
+        else C::store(AD::at(fout, (AI)q * N + cell), f[q]);
       });
     }
   }
@@ -220,12 +221,25 @@ template <typename S, bool A32>
 static void launch_sc(const StepArgs& a, bool store_f, bool flags, bool full, dim3 grid, hipStream_t s) {
   dim3 block(BLOCK);
   if (store_f) {
-    if (flags)
-      hipLaunchKernelGGL((k_stream_collide<S, true, true, true, A32>), grid, block, 0, s, a);
-    else if (full || !A32)
-      hipLaunchKernelGGL((k_stream_collide<S, false, true, true, A32>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((k_stream_collide<S, false, false, true, true>), grid, block, 0, s, a);
+    // streaming stores only in the 32-bit-offset kernels (the output is then
+    // below 4 GiB but far above the last-level cache)
+    const bool nt = A32 && a.stream_store;
+    if (flags) {
+      if (nt)
+        hipLaunchKernelGGL((k_stream_collide<S, true, true, true, true, true>), grid, block, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_stream_collide<S, true, true, true, A32>), grid, block, 0, s, a);
+    } else if (full || !A32) {
+      if (nt)
+        hipLaunchKernelGGL((k_stream_collide<S, false, true, true, true, true>), grid, block, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_stream_collide<S, false, true, true, A32>), grid, block, 0, s, a);
+    } else {
+      if (nt)
+        hipLaunchKernelGGL((k_stream_collide<S, false, false, true, true, true>), grid, block, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_stream_collide<S, false, false, true, true>), grid, block, 0, s, a);
+    }
   } else {
     if (flags)
       hipLaunchKernelGGL((k_stream_collide<S, true, true, false, A32>), grid, block, 0, s, a);

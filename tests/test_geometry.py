@@ -129,6 +129,18 @@ def test_reverse_flags_are_the_transpose_of_gathers(g48):
         assert (kind[zs, ys, xs] != L.POL_KIND_SOLID).all()
 
 
+def test_normal_reader_and_bounce_are_exclusive(g48):
+    """For every non-solid cell x and direction p, f_p(x) is read either by
+    x + e_p (rev bit p) or by x through its bounce link opp(p), never both:
+    the adjoint's reverse gather does one load per direction on that basis."""
+    kind = g48.kind
+    fluid = kind != L.POL_KIND_SOLID
+    for p in range(1, 19):
+        rev_p = (g48.rev_flags >> np.uint32(p)) & np.uint32(1)
+        bounce = (g48.flags >> np.uint32(OPP[p])) & np.uint32(1)
+        assert not np.any(fluid & (rev_p == 1) & (bounce == 1)), p
+
+
 def test_porous_cells_have_les_off(g48):
     por = (g48.flags & L.POL_FLAG_POROUS) != 0
     assert por.any()
