@@ -10,7 +10,9 @@ Forward states are rebuilt from checkpoints every `checkpoint` steps: a
 segment is re-run forward from its checkpoint into a segment store (one
 fused step per state, pol_step_from) and then swept in reverse, so memory
 holds the checkpoints plus one segment (128^3, 50-step segments: 8 GB in
-FP32), far inside 288 GB of HBM.
+FP32), far inside 288 GB of HBM.  With checkpoint=None every forward state is
+kept (128^3 x 500 steps: 80 GB in FP32), the forward run writes them as it
+goes and the reverse sweep recomputes nothing.
 
 There is no adjoint in the reference (no autograd or checkpointing exists in
 src/); OutletFluxFunction wraps the sweep as a torch.autograd.Function so
@@ -31,12 +33,16 @@ from .workloads import V60Workload
 
 
 class AdjointProblem:
-    def __init__(self, wl: V60Workload, K: np.ndarray, steps: int = 500, window: int = 100, checkpoint: int = 50):
+    def __init__(self, wl: V60Workload, K: np.ndarray, steps: int = 500, window: int = 100,
+                 checkpoint: Optional[int] = 50):
         cfg = wl.config
         if not cfg.perm_field:
             raise ValueError("the adjoint workload needs SolverConfig(perm_field=True)")
         if cfg.precision not in ("f32", "f64"):
             raise ValueError("the adjoint runs with f32 or f64 populations")
+        self.store_all = checkpoint is None
+        if self.store_all:
+            checkpoint = steps
         if steps % checkpoint:
             raise ValueError("steps must be a multiple of the checkpoint interval")
         self.wl = wl
@@ -57,8 +63,9 @@ class AdjointProblem:
         L.check(self.ctx.lib.pol_adjoint_bind(self.ctx.handle, C.byref(b)), self.ctx.handle, "pol_adjoint_bind")
         nseg = steps // checkpoint
         sd = cfg.storage_dtype
-        self.ckpt = torch.empty((nseg, 19) + shape, dtype=sd, device=dev)
-        self.seg = torch.empty((checkpoint, 19) + shape, dtype=sd, device=dev)
+        fshape = tuple(self.ctx.f[0].shape)
+        self.ckpt = None if self.store_all else torch.empty((nseg,) + fshape, dtype=sd, device=dev)
+        self.seg = torch.empty((checkpoint,) + fshape, dtype=sd, device=dev)
         self.timings = {}
 
     # ---- parameters -------------------------------------------------------------
@@ -73,11 +80,23 @@ class AdjointProblem:
         return n >= self.steps - self.window
 
     def forward(self, checkpoints: bool = False) -> float:
-        """J (FP64) of one full run from rest; optionally store the checkpoints."""
+        """J (FP64) of one full run from rest; optionally store the checkpoints
+        (or, with checkpoint=None, every state: the run then steps from one
+        stored state into the next)."""
         ctx = self.ctx
         ctx.set_current(0)
         ctx.init_equilibrium()
         J = 0.0
+        if checkpoints and self.store_all:
+            seg = self.seg
+            seg[0].copy_(ctx.f_current)
+            for n in range(self.steps):
+                out = seg[n + 1] if n + 1 < self.steps else ctx.f_next
+                what = L.POL_WANT_REDUCE if self.objective_step(n) else 0
+                red = ctx.step_from(seg[n], out, what)
+                if what:
+                    J += float(red[7])
+            return J
         for n in range(self.steps):
             if checkpoints and n % self.checkpoint == 0:
                 self.ckpt[n // self.checkpoint].copy_(ctx.f_current)
@@ -111,10 +130,11 @@ class AdjointProblem:
         L.check(lib.pol_adjoint_reset(ctx.handle, L.stream_handle()), ctx.handle, "pol_adjoint_reset")
         K = self.checkpoint
         for s in reversed(range(self.steps // K)):
-            # rebuild f^{sK} .. f^{sK+K-1} into the segment store
-            self.seg[0].copy_(self.ckpt[s])
-            for i in range(1, K):
-                ctx.step_from(self.seg[i - 1], self.seg[i])
+            if not self.store_all:
+                # rebuild f^{sK} .. f^{sK+K-1} into the segment store
+                self.seg[0].copy_(self.ckpt[s])
+                for i in range(1, K):
+                    ctx.step_from(self.seg[i - 1], self.seg[i])
             for i in reversed(range(K)):
                 n = s * K + i
                 st = lib.pol_adjoint_step(ctx.handle, L.ptr(self.seg[i]), int(self.objective_step(n)),

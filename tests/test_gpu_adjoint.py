@@ -87,6 +87,18 @@ def test_adjoint_f32_close_to_f64(hip):
     assert abs(u32[2] - u64[2]) <= 1e-3 * abs(u64[2])
 
 
+def test_store_all_equals_checkpointing(hip):
+    """checkpoint=None keeps every forward state; the reverse sweep then
+    recomputes nothing and the gradient is the checkpointed one, bit for bit."""
+    pc, K = _problem(48, "f64", 60, 20, 15)
+    ps, _ = _problem(48, "f64", 60, 20, None)
+    Jc, gc, uc = pc.gradient()
+    Js, gs, us = ps.gradient()
+    assert Jc == Js
+    assert torch.equal(gc, gs)
+    assert np.array_equal(uc, us)
+
+
 def test_autograd_function(hip):
     prob, K = _problem(32, "f64", 30, 10, 10)
     Kt = torch.tensor(K, requires_grad=True)
