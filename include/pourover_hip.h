@@ -136,8 +136,24 @@ typedef struct pol_buffers {
                           the cell (w = trilinear weight, beta = |drag|/|u_rel|).
                           The collision treats the drag B (V - u) implicitly in
                           u, like the Darcy term, so any B is stable. */
+  /* Compact row-run population layout (NULL = dense [19][nz][ny][nx]).
+   * Population q of cell (x, y, z) lives at f[q * stored_cells + slot],
+   * slot = row_base(y, z) + x, for the cells of the row's stored run [x_begin,
+   * x_end) (first to last non-solid cell of the row); rows follow one another
+   * in (z, y) order.  Launches then use one record of POL_RECORD_WORDS uint32
+   * per chunk_cells-cell row chunk meeting a run:
+   *   [0] dense index of the chunk's first cell, [1 + a + 3 b] row_base of
+   *   row (y + a - 1, z + b - 1) (a, b in 0..2; a row past a non-periodic
+   *   face is the chunk's own row, periodic faces wrap), [10] own row's run
+   *   x_begin | x_end << 16, [11] 0.
+   * Every stored slot is written each step (solid cells of a run with zero
+   * deviations).  active_chunks must be NULL with records. */
+  const uint32_t *chunk_records;
+  uint32_t stored_cells; /* plane stride of the compact layout */
+  uint32_t reserved2;
 } pol_buffers;
 #define POL_CHUNK 32 /* default chunk_cells */
+#define POL_RECORD_WORDS 12 /* uint32 per compact-layout chunk record */
 
 /* ---- library ------------------------------------------------------------ */
 /* ABI version (POL_ABI_VERSION). */

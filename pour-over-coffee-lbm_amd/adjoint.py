@@ -34,7 +34,7 @@ from .workloads import V60Workload
 
 class AdjointProblem:
     def __init__(self, wl: V60Workload, K: np.ndarray, steps: int = 500, window: int = 100,
-                 checkpoint: Optional[int] = 50):
+                 checkpoint: Optional[int] = 50, compact: Optional[bool] = None):
         cfg = wl.config
         if not cfg.perm_field:
             raise ValueError("the adjoint workload needs SolverConfig(perm_field=True)")
@@ -47,12 +47,13 @@ class AdjointProblem:
             raise ValueError("steps must be a multiple of the checkpoint interval")
         self.wl = wl
         self.steps, self.window, self.checkpoint = steps, window, checkpoint
-        self.ctx = HipContext(cfg, wl.geometry.flags)
+        self.ctx = HipContext(cfg, wl.geometry.flags, compact=compact)
         self.ctx.perm.copy_(torch.as_tensor(K, dtype=cfg.real_dtype))
         shape = self.ctx.shape
         rd = cfg.real_dtype
         dev = self.ctx.device
-        self.lam = [torch.zeros((19,) + shape, dtype=rd, device=dev) for _ in range(2)]
+        # adjoint populations in the forward state's storage layout
+        self.lam = [torch.zeros(tuple(self.ctx.f[0].shape), dtype=rd, device=dev) for _ in range(2)]
         self.grad_perm = torch.zeros(shape, dtype=torch.float64, device=dev)
         self.rev_flags = torch.from_numpy(np.ascontiguousarray(wl.geometry.rev_flags).view(np.int32)).to(dev)
         b = L.PolAdjointBuffers()

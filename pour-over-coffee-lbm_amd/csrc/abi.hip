@@ -19,6 +19,7 @@ struct pol_ctx {
   Grid grid{};   // dense grid over all cells (sizes the partial buffers)
   Grid wgrid{};  // work grid: over the active-cell list when bound, else = grid
   uint32_t cshift = 5;    // log2 of the cells per chunk of the bound list
+  uint32_t P = 0;         // population plane stride (N, or the compact layout's stored cells)
   uint32_t part_cap = 0;  // blocks the partial buffers hold (any work grid fits)
   double* partial = nullptr;  // [nblk][POL_NRED]
   double* red_dev = nullptr;  // [POL_NRED]
@@ -147,16 +148,23 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
     return fail(c, POL_ERR_INVALID, "pol_bind: chunk_cells must be 8, 16, 32 or 64");
   const uint32_t cshift = cc == 8 ? 3 : (cc == 16 ? 4 : (cc == 32 ? 5 : 6));
   const uint64_t max_chunks = (uint64_t)c->d.ny * c->d.nz * (((uint64_t)c->d.nx + cc - 1) / cc);
-  if (b->active_chunks &&
-      (b->n_chunks == 0 || b->n_chunks > max_chunks || ((uint64_t)b->n_chunks << cshift) >= (1ull << 32)))
+  const bool listed = b->active_chunks || b->chunk_records;
+  if (b->active_chunks && b->chunk_records)
+    return fail(c, POL_ERR_INVALID, "pol_bind: pass either active_chunks (dense layout) or chunk_records (compact)");
+  if (listed && (b->n_chunks == 0 || b->n_chunks > max_chunks || ((uint64_t)b->n_chunks << cshift) >= (1ull << 32)))
     return fail(c, POL_ERR_INVALID, "pol_bind: n_chunks out of range");
-  if (b->active_chunks && !(o & POL_USE_FLAGS))
+  if (listed && !(o & POL_USE_FLAGS))
     return fail(c, POL_ERR_INVALID, "pol_bind: a chunk list needs cell flags (solid lanes are masked by them)");
-  const Grid wg = b->active_chunks ? make_grid((uint64_t)b->n_chunks << cshift) : c->grid;
+  if (b->chunk_records && (b->stored_cells == 0 || b->stored_cells >= (1u << 31)))
+    return fail(c, POL_ERR_INVALID, "pol_bind: compact layout needs 0 < stored_cells < 2^31");
+  if (b->chunk_records && ((uintptr_t)b->chunk_records & 15))
+    return fail(c, POL_ERR_INVALID, "pol_bind: chunk_records must be 16-byte aligned");
+  const Grid wg = listed ? make_grid((uint64_t)b->n_chunks << cshift) : c->grid;
   if (wg.nblk > c->part_cap) return fail(c, POL_ERR_INVALID, "pol_bind: work grid exceeds the reduction scratch");
   c->b = *b;
   c->b.chunk_cells = cc;
   c->cshift = cshift;
+  c->P = b->chunk_records ? b->stored_cells : c->N;
   if (!(o & POL_USE_FLAGS)) c->b.cell_flags = nullptr;
   c->wgrid = wg;
   c->bound = true;
@@ -187,11 +195,16 @@ int pol_set_current(pol_ctx* c, int index) {
   return POL_OK;
 }
 
+static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_t what);
+
 int pol_init_equilibrium(pol_ctx* c, const void* rho, const void* u, void* stream) {
   if (!c) return POL_ERR_INVALID;
   if (!c->bound) return fail(c, POL_ERR_STATE, "pol_init_equilibrium: buffers not bound");
   hipError_t e = hipSetDevice(c->d.device);
-  if (e == hipSuccess) e = launch_init_equilibrium(c->d.storage, c->b.f[c->cur], rho, u, c->N, (hipStream_t)stream);
+  if (e == hipSuccess) {
+    const StepArgs a = make_args(c, nullptr, c->b.f[c->cur], 0);
+    e = launch_init_equilibrium(c->d.storage, a, rho, u, c->wgrid, (hipStream_t)stream);
+  }
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_init_equilibrium");
 }
 
@@ -215,7 +228,9 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.dnx = make_fastdiv(d.nx);
   a.dny = make_fastdiv(d.ny);
   a.chunks = c->b.active_chunks;
-  a.n_items = c->b.active_chunks ? c->b.n_chunks << c->cshift : c->N;
+  a.records = c->b.chunk_records;
+  a.P = c->P;
+  a.n_items = (c->b.active_chunks || c->b.chunk_records) ? c->b.n_chunks << c->cshift : c->N;
   a.cshift = c->cshift;
   // Streaming (non-temporal) stores once a step's population traffic (read +
   // write) exceeds the 256 MiB last-level cache: at 224^3 they cut the V60
@@ -403,8 +418,8 @@ int pol_adjoint_reset(pol_ctx* c, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const size_t rb = c->d.storage == POL_F64 ? 8 : 4;
   hipError_t e = hipSetDevice(c->d.device);
-  if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[0], 0, (size_t)POL_Q * c->N * rb, s);
-  if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[1], 0, (size_t)POL_Q * c->N * rb, s);
+  if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[0], 0, (size_t)POL_Q * c->P * rb, s);
+  if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[1], 0, (size_t)POL_Q * c->P * rb, s);
   if (e == hipSuccess && c->ab.grad_perm) e = hipMemsetAsync(c->ab.grad_perm, 0, (size_t)c->N * sizeof(double), s);
   if (e == hipSuccess) e = hipMemsetAsync(c->adj_acc, 0, 4 * sizeof(double), s);
   c->acur = 0;

@@ -75,7 +75,7 @@ struct AdjGeom {
   bool px, py, pz;
 };
 
-template <typename S, typename R, bool A32>
+template <typename S, typename R, bool A32, bool SPARSE>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const uint32_t* __restrict__ rflags) {
   using Cd = Codec<S>;
   using AD = Addr<A32>;
@@ -84,8 +84,18 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   const uint32_t lblk = logical_block(a.per_xcd);
   if (lblk >= a.nblk) return;
   uint32_t cell, yz, x;
-  const bool inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
-  const uint32_t N = a.N;
+  int rb[9];  // compact layout: slot bases of the nine neighbour rows
+  bool in_run = false;
+  bool inside;
+  if constexpr (SPARSE)
+    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz, x,
+                         rb, in_run);
+  else
+    inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
+  const uint32_t N = a.N;  // dense cells
+  const uint32_t P = a.P;  // population plane stride
+  uint32_t own = cell;     // population slot of this cell
+  if constexpr (SPARSE) own = slot_clamp(rb[4] + (int)x, P);
   const uint32_t z = fdiv(yz, a.dny);
   const uint32_t y = yz - z * a.ny;
   const uint32_t fl = inside ? a.flags[cell] : (POL_KIND_SOLID << POL_KIND_SHIFT);
@@ -129,7 +139,14 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       const bool nb = rf & (1u << p);
       bool bb = false;
       if constexpr (p != 0) bb = kind != POL_KIND_INLET && (fl & (1u << OPP(p)));
-      const AI src = nb ? (AI)p * N + (uint32_t)((int)cell + off) : (AI)OPP(p) * N + cell;
+      uint32_t rslot;
+      if constexpr (SPARSE) {
+        const int dx = EX(p) > 0 ? oxp : (EX(p) < 0 ? oxm : 0);
+        rslot = slot_clamp(rb[(1 + EY(p)) + 3 * (1 + EZ(p))] + (int)x + dx, P);
+      } else {
+        rslot = (uint32_t)((int)cell + off);
+      }
+      const AI src = nb ? (AI)p * P + rslot : (AI)OPP(p) * P + own;
       const R v = *AD::at(mu_in, src);
       lam[p] = (nb || bb) ? v : R(0);
     });
@@ -163,7 +180,13 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           const uint32_t rfl = a.flags[rc];
           const uint32_t rk = (rfl >> POL_KIND_SHIFT) & POL_KIND_MASK;
           if (rk == POL_KIND_SOLID || rk == POL_KIND_INLET || (rfl & (1u << p))) continue;
-          lam[p] += *AD::at(mu_in, (AI)p * N + rc);
+          uint32_t rs = rc;
+          if constexpr (SPARSE) {
+            // the reader's row is one of the record's nine rows
+            const int ra = (sub & 2) ? 1 : 1 + e[1], rbz = (sub & 4) ? 1 : 1 + e[2];
+            rs = slot_clamp(rb[ra + 3 * rbz] + r[0], P);
+          }
+          lam[p] += *AD::at(mu_in, (AI)p * P + rs);
         }
       });
     }
@@ -192,9 +215,15 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         constexpr int q = decltype(qc)::value;
         const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
                         (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
-        AI src = (AI)q * N + (uint32_t)((int)cell + off);
+        AI src;
+        if constexpr (SPARSE) {
+          const int dx = EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0);
+          src = (AI)q * P + slot_clamp(rb[(1 - EY(q)) + 3 * (1 - EZ(q))] + (int)x + dx, P);
+        } else {
+          src = (AI)q * P + (uint32_t)((int)cell + off);
+        }
         if constexpr (q != 0) {
-          if (fl & (1u << q)) src = (AI)OPP(q) * N + cell;
+          if (fl & (1u << q)) src = (AI)OPP(q) * P + own;
         }
         const R d = Cd::load(AD::at(fin, src));
         s36 += d;
@@ -394,10 +423,12 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   {
     // One store loop for every lane kind.  Solid lanes of a launched chunk
     // store too (zeros), so every 128-byte output line is written whole;
-    // dense launches do so only for 32-lane groups that hold a non-solid cell.
+    // dense launches do so only for 32-lane groups that hold a non-solid cell;
+    // the compact layout stores exactly the slots of the stored runs.
     const uint64_t any = __ballot(active);
     const uint64_t group = 0xffffffffull << (threadIdx.x & 32);
-    if (inside && (active || a.chunks != nullptr || (any & group) != 0)) {
+    const bool store = SPARSE ? in_run : (inside && (active || a.chunks != nullptr || (any & group) != 0));
+    if (store) {
       R* __restrict__ mu_out = (R*)A.lam_out;
       static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -408,7 +439,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         eacc<EX(q) * EY(q)>(ePe, Pxy2);
         eacc<EX(q) * EZ(q)>(ePe, Pxz2);
         eacc<EY(q) * EZ(q)>(ePe, Pyz2);
-        *AD::at(mu_out, (AI)q * N + cell) = keep * lam[q] + C + edot<EX(q), EY(q), EZ(q)>(hx, hy, hz) + L1 * ePe;
+        *AD::at(mu_out, (AI)q * P + own) = keep * lam[q] + C + edot<EX(q), EY(q), EZ(q)>(hx, hy, hz) + L1 * ePe;
       });
     }
   }
@@ -443,20 +474,25 @@ __global__ void __launch_bounds__(256) k_adjoint_final(const double* __restrict_
   if (threadIdx.x < 4) acc[threadIdx.x] += sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
 
+template <typename S, bool SPARSE>
+static void launch_adj(const AdjointArgs& A, const uint32_t* rflags, dim3 grid, hipStream_t s) {
+  if (fits_addr32(Q, A.fw.P, sizeof(S)))
+    hipLaunchKernelGGL((k_adjoint<S, S, true, SPARSE>), grid, dim3(BLOCK), 0, s, A, rflags);
+  else
+    hipLaunchKernelGGL((k_adjoint<S, S, false, SPARSE>), grid, dim3(BLOCK), 0, s, A, rflags);
+}
+
 hipError_t launch_adjoint(int storage, const AdjointArgs& A, const uint32_t* rflags, const Grid& g, hipStream_t s) {
-  dim3 grid(g.launch), block(BLOCK);
+  const dim3 grid(g.launch);
+  const bool sparse = A.fw.records != nullptr;
   switch (storage) {
     case POL_F32:
-      if (fits_addr32(Q, A.fw.N, sizeof(float)))
-        hipLaunchKernelGGL((k_adjoint<float, float, true>), grid, block, 0, s, A, rflags);
-      else
-        hipLaunchKernelGGL((k_adjoint<float, float, false>), grid, block, 0, s, A, rflags);
+      if (sparse) launch_adj<float, true>(A, rflags, grid, s);
+      else launch_adj<float, false>(A, rflags, grid, s);
       break;
     case POL_F64:
-      if (fits_addr32(Q, A.fw.N, sizeof(double)))
-        hipLaunchKernelGGL((k_adjoint<double, double, true>), grid, block, 0, s, A, rflags);
-      else
-        hipLaunchKernelGGL((k_adjoint<double, double, false>), grid, block, 0, s, A, rflags);
+      if (sparse) launch_adj<double, true>(A, rflags, grid, s);
+      else launch_adj<double, false>(A, rflags, grid, s);
       break;
     default: return hipErrorInvalidValue;
   }

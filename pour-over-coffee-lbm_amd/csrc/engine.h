@@ -20,6 +20,8 @@ struct StepArgs {
   void* u;            // real [3][N]
   double* partial;    // [nblk][POL_NRED]
   const uint32_t* chunks;  // chunk list: item -> chunks[item >> cshift] + lane (null: item = cell)
+  const uint32_t* records; // compact layout: chunk records (POL_RECORD_WORDS each), else null
+  uint32_t P;              // population plane stride: N (dense) or the compact layout's stored cells
   uint32_t n_items;        // number of work items (cells, or chunk cells x chunks)
   uint32_t cshift;         // log2 of the cells per chunk
   uint32_t stream_store;   // select the non-temporal-store kernels (output far above the last-level cache)
@@ -39,8 +41,10 @@ struct StepArgs {
 hipError_t launch_stream_collide(int storage, const StepArgs& a, bool store_f, const Grid& g, hipStream_t s);
 // Fold per-block partials into out[POL_NRED] (device), deterministic order.
 hipError_t launch_reduce_final(const double* partial, uint32_t nblk, double* out, hipStream_t s);
-// f = feq(rho, u) (rho/u real-typed, may be null)
-hipError_t launch_init_equilibrium(int storage, void* f, const void* rho, const void* u, uint32_t N, hipStream_t s);
+// f = feq(rho, u) (rho/u real-typed, may be null); a.records selects the
+// compact layout (a.fout = f, a.N, a.P, a.n_items, a.cshift, a.dnx, a.nx used)
+hipError_t launch_init_equilibrium(int storage, const StepArgs& a, const void* rho, const void* u, const Grid& g,
+                                   hipStream_t s);
 
 // Particle state in the real type (float, or double with real_is_double).
 struct ParticleArgs {
@@ -65,7 +69,7 @@ hipError_t launch_particle_index(const float* pos, int32_t n, int32_t nx, int32_
 
 struct AdjointArgs {
   StepArgs fw;          // forward description; fw.fin = forward state f^n
-  const void* lam_in;   // adjoint of f^{n+1} (post-collision), [19][N]
+  const void* lam_in;   // adjoint of f^{n+1} (post-collision), [19][P]
   void* lam_out;        // adjoint of f^n
   double* grad_perm;    // [N]
   double* partial;      // [nblk][4]: dJ/du_in x,y,z and J

@@ -13,13 +13,19 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from .layout import build_compact_layout
 
 POL_CHUNK = L.POL_CHUNK
 WRITE_GRANULE = 128  # bytes: HBM writes narrower than this line cost a read-modify-write
-# cells per launch chunk by storage (measured at 224^3: FP32 16 / 32 / 64-cell
-# chunks 0.174 / 0.151 / 0.159 ms, 16-bit 16 / 32 / 64 cells 0.114 / 0.103 /
-# 0.110 ms)
+# Population layout and chunk size by storage, from the 224^3 V60 step:
+#   dense layout, FP32 16 / 32 / 64-cell chunks: 0.174 / 0.145 / 0.159 ms
+#   compact layout, FP32 16 / 32 / 64-cell chunks: 0.166 / 0.154 / 0.143 ms
+#     (15 % fewer HBM bytes, but unaligned runs cost as much in requests)
+#   dense layout, 16-bit 32-cell chunks: 0.098 ms
+#   compact layout, 16-bit 16 / 32 / 64-cell chunks: 0.091 / 0.093 / 0.100 ms
+COMPACT_DEFAULT = {"f32": False, "f16": True, "bf16": True, "f64": False}
 CHUNK_CELLS = {"f32": 32, "f16": 32, "bf16": 32, "f64": 16}
+CHUNK_CELLS_COMPACT = {"f32": 64, "f16": 16, "bf16": 16, "f64": 32}
 _STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 
 
@@ -109,7 +115,10 @@ def _as_int32_view(a: np.ndarray) -> torch.Tensor:
 class HipContext:
     """One pol_ctx with its device buffers."""
 
-    def __init__(self, cfg: SolverConfig, flags: Optional[np.ndarray] = None, chunk_cells: Optional[int] = None):
+    def __init__(self, cfg: SolverConfig, flags: Optional[np.ndarray] = None, chunk_cells: Optional[int] = None,
+                 compact: Optional[bool] = None):
+        """compact: store populations in the row-run layout (layout.py); by
+        default on for 16-bit storage whenever cell flags are given."""
         if not torch.cuda.is_available():
             raise RuntimeError("HipContext needs a visible HIP device (torch.cuda.is_available() is False)")
         self.lib = L.load()
@@ -119,13 +128,17 @@ class HipContext:
         self.shape = shape
         self.cells = cfg.cells
         sd, rd = cfg.storage_dtype, cfg.real_dtype
-        # zero = rest state f = w; solid cells of the write buffer are never written
-        self.f = [torch.zeros((19,) + shape, dtype=sd, device=self.device) for _ in range(2)]
         self.rho = torch.ones(shape, dtype=rd, device=self.device)
         self.u = torch.zeros((3,) + shape, dtype=rd, device=self.device)
         self.flags = _as_int32_view(flags).to(self.device) if flags is not None else None
-        self.chunk_cells = int(chunk_cells or CHUNK_CELLS[cfg.precision])
-        self.chunks = self._chunk_list(flags)
+        self.compact = (flags is not None and COMPACT_DEFAULT[cfg.precision]) if compact is None else bool(compact)
+        table = CHUNK_CELLS_COMPACT if self.compact else CHUNK_CELLS
+        self.chunk_cells = int(chunk_cells or table[cfg.precision])
+        if self.compact and flags is None:
+            raise ValueError("the compact layout needs cell flags")
+        self._set_layout(flags)
+        # zero = rest state f = w
+        self.f = [torch.zeros((19, self.plane), dtype=sd, device=self.device) for _ in range(2)]
         self.force = torch.zeros((3,) + shape, dtype=rd, device=self.device) if cfg.force_field else None
         self.perm = torch.full(shape, cfg.darcy_K, dtype=rd, device=self.device) if cfg.perm_field else None
         self.pdrag = torch.zeros((4,) + shape, dtype=rd, device=self.device) if cfg.particle_field else None
@@ -145,12 +158,55 @@ class HipContext:
         b.perm = self.perm.data_ptr() if self.perm is not None else None
         b.rho = self.rho.data_ptr()
         b.u = self.u.data_ptr()
-        if self.chunks is not None:
+        if self.records is not None:
+            b.chunk_records = self.records.data_ptr()
+            b.n_chunks = self.layout.n_chunks
+            b.stored_cells = self.layout.stored_cells
+            b.chunk_cells = self.chunk_cells
+        elif self.chunks is not None:
             b.active_chunks = self.chunks.data_ptr()
             b.n_chunks = self.chunks.numel()
             b.chunk_cells = self.chunk_cells
         b.particle_drag = self.pdrag.data_ptr() if self.pdrag is not None else None
         L.check(self.lib.pol_bind(self._ctx, C.byref(b)), self._ctx, "pol_bind")
+
+    def _set_layout(self, flags: Optional[np.ndarray]) -> None:
+        """Launch list and population layout for these flags: the compact
+        row-run layout with chunk records, or the dense layout with (when
+        flags are given) a dense chunk list."""
+        cfg = self.config
+        if self.compact:
+            self.layout = build_compact_layout(flags, tuple(bool(p) for p in cfg.periodic), self.chunk_cells)
+            self.records = _as_int32_view(self.layout.records.reshape(-1)).to(self.device)
+            self.chunks = None
+            self.plane = self.layout.stored_cells
+        else:
+            self.layout = None
+            self.records = None
+            self.chunks = self._chunk_list(flags)
+            self.plane = cfg.cells
+
+    def dense_populations(self, t: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Populations [19, nz, ny, nx] of a state buffer (default: the current
+        one) in the dense layout; cells outside the stored runs read 0 (= w)."""
+        t = self.f_current if t is None else t
+        if self.layout is None:
+            return t.view((19,) + self.shape)
+        idx = torch.from_numpy(self.layout.dense_index).to(self.device)
+        out = torch.zeros((19, self.cells), dtype=t.dtype, device=self.device)
+        out[:, idx] = t[:, :self.layout.used_cells]
+        return out.view((19,) + self.shape)
+
+    def load_dense_populations(self, dense: torch.Tensor, t: Optional[torch.Tensor] = None) -> None:
+        """Write dense populations [19, nz, ny, nx] into a state buffer."""
+        t = self.f_current if t is None else t
+        d = dense.to(self.device, t.dtype).reshape(19, -1)
+        if self.layout is None:
+            t.copy_(d)
+            return
+        idx = torch.from_numpy(self.layout.dense_index).to(self.device)
+        t.zero_()
+        t[:, :self.layout.used_cells] = d[:, idx]
 
     def _chunk_list(self, flags: Optional[np.ndarray]) -> Optional[torch.Tensor]:
         """Launch list: first cells of the chunk_cells-cell row chunks holding
@@ -216,9 +272,16 @@ class HipContext:
     def set_flags(self, flags: np.ndarray) -> None:
         if not self._use_flags:
             raise ValueError("context was created without cell flags")
-        self.flags.copy_(_as_int32_view(flags).to(self.device))
-        self.chunks = self._chunk_list(flags)
         cur = self.current
+        dense = [self.dense_populations(t) for t in self.f] if self.compact else None
+        self.flags.copy_(_as_int32_view(flags).to(self.device))
+        self._set_layout(flags)
+        if dense is not None:
+            # the stored runs follow the flags: remap both states
+            sd = self.config.storage_dtype
+            self.f = [torch.zeros((19, self.plane), dtype=sd, device=self.device) for _ in range(2)]
+            for t, d in zip(self.f, dense):
+                self.load_dense_populations(d, t)
         self._bind()
         self.set_current(cur)
 

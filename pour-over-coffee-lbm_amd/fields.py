@@ -41,17 +41,23 @@ class DeviceField:
     """View of a device tensor shaped [C..., nz, ny, nx] as a reference field."""
 
     def __init__(self, tensor: torch.Tensor, kind: str = "scalar", before_read: Optional[Callable[[], None]] = None,
-                 after_write: Optional[Callable[[], None]] = None, deviation: bool = False):
+                 after_write: Optional[Callable[[], None]] = None, deviation: bool = False, ctx=None):
         assert kind in ("scalar", "vector", "population")
         self._t = tensor
         self.kind = kind
         self._before_read = before_read
         self._after_write = after_write
         self._deviation = deviation  # population storage holds f - w
+        # populations: the HipContext that maps its storage layout (dense or
+        # compact row runs) to [19, nz, ny, nx]
+        self._ctx = ctx
 
     # -- shape / dtype -----------------------------------------------------
     @property
     def shape(self):
+        if self._ctx is not None:
+            nz, ny, nx = self._ctx.shape
+            return (self._t.shape[0], nx, ny, nz)
         nz, ny, nx = self._t.shape[-3:]
         if self.kind == "scalar":
             return (nx, ny, nz)
@@ -68,7 +74,8 @@ class DeviceField:
         return self._t.dtype
 
     def to_torch(self) -> torch.Tensor:
-        """The underlying device tensor in device layout ([C,] nz, ny, nx)."""
+        """The underlying device tensor in device layout ([C,] nz, ny, nx; a
+        population buffer in its storage layout, [19, slots])."""
         if self._before_read:
             self._before_read()
         return self._t
@@ -77,7 +84,7 @@ class DeviceField:
     def to_numpy(self) -> np.ndarray:
         if self._before_read:
             self._before_read()
-        t = self._t
+        t = self._t if self._ctx is None else self._ctx.dense_populations(self._t)
         if self._deviation:
             a = decode_populations(t)
         else:
@@ -99,13 +106,16 @@ class DeviceField:
             if self._deviation:
                 dev = encode_populations(dev)
         src = torch.from_numpy(np.ascontiguousarray(dev)).to(self._t.dtype)
-        self._t.copy_(src.to(self._t.device))
+        if self._ctx is not None:
+            self._ctx.load_dense_populations(src, self._t)
+        else:
+            self._t.copy_(src.to(self._t.device))
         if self._after_write:
             self._after_write()
 
     def fill(self, value) -> None:
         if self.kind == "population" and self._deviation:
-            w = torch.tensor(W, dtype=torch.float64, device=self._t.device).view(-1, 1, 1, 1)
+            w = torch.tensor(W, dtype=torch.float64, device=self._t.device).view((-1,) + (1,) * (self._t.dim() - 1))
             self._t.copy_((SCALE * (float(value) - w)).to(self._t.dtype).expand_as(self._t))
         elif self.kind == "vector" and not np.isscalar(value):
             v = torch.tensor(list(value), dtype=self._t.dtype, device=self._t.device).view(3, 1, 1, 1)

@@ -138,7 +138,7 @@ class HipLBMSolver:
         # reference-style field views
         self.rho = DeviceField(self.ctx.rho, "scalar", before_read=self._ensure_macro)
         self.u = DeviceField(self.ctx.u, "vector", before_read=self._ensure_macro)
-        self._f_views = [DeviceField(t, "population", after_write=self._invalidate, deviation=dev16)
+        self._f_views = [DeviceField(t, "population", after_write=self._invalidate, deviation=dev16, ctx=self.ctx)
                          for t in self.ctx.f]
         self.body_force = DeviceField(self.ctx.force, "vector") if self.ctx.force is not None else None
         self.phase = DeviceField(torch.ones(self.ctx.shape, dtype=config.real_dtype, device=self.ctx.device),
@@ -262,6 +262,9 @@ class HipLBMSolver:
         if self._flags_host is None:
             raise ValueError("solver was built without cell flags; pass the geometry at construction")
         self.ctx.set_flags(flags)
+        # the compact layout follows the flags: the state buffers are new
+        self._f_views = [DeviceField(t, "population", after_write=self._invalidate, deviation=True, ctx=self.ctx)
+                         for t in self.ctx.f]
         self._flags_host = flags
         kind = (flags >> L.POL_KIND_SHIFT) & L.POL_KIND_MASK
         self._solid_t.copy_(torch.from_numpy((kind == L.POL_KIND_SOLID).astype(np.uint8)))
