@@ -5,7 +5,8 @@ domain in FP32 -- 30 deg cone, outlet hole radius 8, 2-cell porous filter
 (Darcy K 1e-3, Forchheimer 0.55), Guo gravity 1e-5, Smagorinsky 0.17,
 inlet jet radius 4 at u_z = -0.03.  Populations start at rest and are
 resident in HBM before timing.  `value` = fluid-cell updates per second
-(MLUPS); solid cells (67 %) are neither read nor written.
+(MLUPS); the 67 % of cells that are solid get no threads except the solid
+tails of launched 32-cell row chunks.
 
 Secondary lines in the same JSON object: the all-fluid periodic 224^3 box
 (every cell updated), FP16/BF16 storage speed-up over FP32, adjoint cost per
@@ -65,10 +66,11 @@ def bench_v60(precision, steps, warmup):
     ev_s, wall = time_steps(ctx, steps, warmup)
     red = ctx.macroscopic(macro=False, reduce=True)
     active = counts["fluid"] + counts["inlet"] + counts["outlet"]
+    layout = "compact row runs" if ctx.compact else "dense"
     del ctx
     torch.cuda.empty_cache()
     return {"event_s": ev_s, "wall_s": wall, "active_cells": active, "cells": 224**3, "red": red.tolist(),
-            "counts": counts}
+            "counts": counts, "layout": layout}
 
 
 def bench_periodic(precision, steps, warmup, n=224):
@@ -297,7 +299,8 @@ def main():
         "vs_baseline": None,
         "dtype": prec,
         "data": "synthetic (V60 geometry generated in-tree, fluid at rest + inlet jet)",
-        "config": {"workload": "v60_224_%s" % prec, "grid": [224, 224, 224], "active_cells": r["active_cells"],
+        "config": {"workload": "v60_224_%s" % prec, "layout": r["layout"], "grid": [224, 224, 224],
+                   "active_cells": r["active_cells"],
                    "solid_cells": r["counts"]["solid"], "porous_cells": r["counts"]["porous"], "tau": 0.6,
                    "les_cs": 0.17, "gravity": 1e-5, "darcy_K": 1e-3, "forch_CF": 0.55, "inlet_u": -0.03,
                    "parallelism": "single GPU"},
@@ -321,7 +324,9 @@ def main():
             s16 = r16["event_s"] / args.steps
             out["%s_storage" % p16] = {"mlups": round(r16["active_cells"] / s16 / 1e6, 1),
                                        "kernel_ms": round(s16 * 1e3, 4), "speedup_vs_f32": round(k_s / s16, 3),
-                                       "achieved_GBs": round(76 * r16["active_cells"] / s16 / 1e9, 1)}
+                                       "achieved_GBs": round(76 * r16["active_cells"] / s16 / 1e9, 1),
+                                       "frac": round(76 * r16["active_cells"] / s16 / 1e9 / HBM_PEAK_GBS, 4),
+                                       "layout": r16["layout"], "traffic": counter_traffic(p16)}
         log("bench: adjoint 128^3 f32")
         out["adjoint"] = bench_adjoint()
         log("bench: cpu baseline")
