@@ -9,9 +9,9 @@ resident in HBM before timing.  `value` = fluid-cell updates per second
 tails of launched 32-cell row chunks.
 
 Secondary lines in the same JSON object: the all-fluid periodic 224^3 box
-(every cell updated), FP16/BF16 storage speed-up over FP32, adjoint cost per
-forward step, and the CPU baseline (the oracle's C restatement on the host
-cores, bounded sample).
+(every cell updated), FP16/BF16 storage speed-up over FP32, the two-way
+coupled step with 50 000 particles, adjoint cost per forward step, and the
+CPU baseline (C restatements on the host cores, bounded samples).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -87,6 +87,39 @@ def bench_periodic(precision, steps, warmup, n=224):
     del ctx
     torch.cuda.empty_cache()
     return {"event_s": ev_s, "wall_s": wall, "cells": n**3}
+
+
+def bench_coupled(precision, steps, warmup, count=50_000):
+    """Two-way coupled step of the 224^3 cup with the 50 000-particle seed:
+    particle coupling, fused stream-collide (writing rho/u for the next
+    interpolation) and particle advance, all in one native loop."""
+    import torch
+
+    from pourover_amd import workloads as WL
+    from pourover_amd.engine import HipContext
+    from pourover_amd.particles import HipCoffeeParticleSystem, particle_gravity
+
+    wl = WL.v60_config(224, precision, particle_field=True)
+    seedp = WL.seed_particles(wl.geometry, count)
+    ctx = HipContext(wl.config, wl.geometry.flags)
+    ctx.init_equilibrium()
+    ps = HipCoffeeParticleSystem(ctx, seedp.pos, seedp.radius, seedp.mass, gravity=particle_gravity(1e-5, 1.2))
+    ps.prime()
+    ps.coupled_step(nsteps=warmup)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    ps.coupled_step(nsteps=steps)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    counts = wl.geometry.counts()
+    active = counts["fluid"] + counts["inlet"] + counts["outlet"]
+    n_active = int(ps.active.sum().item())
+    del ps, ctx
+    torch.cuda.empty_cache()
+    return {"ms_per_step": round(ms, 4), "mlups": round(active / ms / 1e3, 1), "particles": count,
+            "active_particles_after": n_active, "precision": precision}
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -327,6 +360,8 @@ def main():
                                        "achieved_GBs": round(76 * r16["active_cells"] / s16 / 1e9, 1),
                                        "frac": round(76 * r16["active_cells"] / s16 / 1e9 / HBM_PEAK_GBS, 4),
                                        "layout": r16["layout"], "traffic": counter_traffic(p16)}
+        log("bench: coupled 224^3 f32 + 50k particles")
+        out["coupled_224_f32_50k_particles"] = bench_coupled("f32", args.steps, args.warmup)
         log("bench: adjoint 128^3 f32")
         out["adjoint"] = bench_adjoint()
         log("bench: cpu baseline")

@@ -261,6 +261,13 @@ int pol_particles_couple(pol_ctx *ctx, void *stream);
  * Replaces clear_reaction_forces + update_particle_physics
  * (coffee_particles.py:1101-1105,641-720). */
 int pol_particles_advance(pol_ctx *ctx, void *stream);
+/* nsteps two-way coupled steps, each: pol_particles_couple, one fused
+ * stream-collide step writing rho/u (the velocity the next coupling
+ * interpolates), pol_particles_advance.  The last step also fills red_out
+ * with `what & POL_WANT_REDUCE` (synchronous then).  Replaces the per-step
+ * loop of LBMSolver.step_with_two_way_coupling (legacy/lbm_solver.py:1485-1509)
+ * and main.py's coupling order (main.py:795-839). */
+int pol_coupled_step(pol_ctx *ctx, int nsteps, uint32_t what, double *red_out, void *stream);
 /* Particle -> cell index and clamped fractions, the formula of
  * coffee_particles.py:1159-1174: i = (int)max(0, min(n-2, p)),
  * frac = clamp(p - i, 0, 1).  pos SoA [3][n]; idx [3][n]; frac [3][n]. */

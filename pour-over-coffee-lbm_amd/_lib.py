@@ -110,6 +110,7 @@ SIGNATURES = {
     "pol_particles_bind": (C.c_int, [_P, C.POINTER(PolParticleDesc), C.POINTER(PolParticleBuffers)]),
     "pol_particles_couple": (C.c_int, [_P, _P]),
     "pol_particles_advance": (C.c_int, [_P, _P]),
+    "pol_coupled_step": (C.c_int, [_P, C.c_int, C.c_uint32, _P, _P]),
     "pol_particle_index": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P]),
     "pol_adjoint_bind": (C.c_int, [_P, C.POINTER(PolAdjointBuffers)]),
     "pol_adjoint_step": (C.c_int, [_P, _P, C.c_int, _P]),

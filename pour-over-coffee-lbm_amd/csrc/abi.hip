@@ -326,6 +326,7 @@ int pol_reduce(pol_ctx* c, double* red_out, void* stream) {
 int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particle_buffers* b) {
   if (!c || !d || !b) return POL_ERR_INVALID;
   if (d->count < 0) return fail(c, POL_ERR_INVALID, "pol_particles_bind: negative count");
+  if (d->count > (1 << 27)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: at most 2^27 particles (8 lanes each)");
   if (!c->bound || !(c->d.options & POL_PARTICLE_FIELD))
     return fail(c, POL_ERR_STATE, "pol_particles_bind: bind forward buffers with POL_PARTICLE_FIELD first");
   if (d->count > 0 && (!b->pos || !b->vel || !b->radius || !b->mass || !b->active || !b->drag_new || !b->drag_old ||
@@ -385,6 +386,31 @@ int pol_particles_advance(pol_ctx* c, void* stream) {
   hipError_t e = hipSetDevice(c->d.device);
   if (e == hipSuccess) e = launch_particles_advance(c->pa, (hipStream_t)stream);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_advance");
+}
+
+int pol_coupled_step(pol_ctx* c, int nsteps, uint32_t what, double* red_out, void* stream) {
+  if (!c || nsteps < 0) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_coupled_step: buffers not bound");
+  if (!c->particles) return fail(c, POL_ERR_STATE, "pol_coupled_step: no particles bound");
+  if (!c->b.rho || !c->b.u) return fail(c, POL_ERR_INVALID, "pol_coupled_step: the coupling needs the rho/u buffers");
+  int st = check_what(c, what, red_out, "pol_coupled_step");
+  if (st) return st;
+  if (nsteps == 0) return POL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
+    e = launch_particles_couple(c->pa, s);
+    if (e != hipSuccess) break;
+    const uint32_t w = POL_WANT_MACRO | ((i == nsteps - 1) ? what : 0u);
+    StepArgs a = make_args(c, c->b.f[c->cur], c->b.f[1 - c->cur], w);
+    e = launch_stream_collide(c->d.storage, a, true, c->wgrid, s);
+    if (e != hipSuccess) break;
+    c->cur = 1 - c->cur;
+    e = launch_particles_advance(c->pa, s);
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "pol_coupled_step");
+  if (what & POL_WANT_REDUCE) return finish_reduce(c, red_out, s, "pol_coupled_step");
+  return POL_OK;
 }
 
 int pol_particle_index(const float* pos, int32_t n, int32_t nx, int32_t ny, int32_t nz, int32_t* idx, float* frac,

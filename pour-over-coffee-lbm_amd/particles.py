@@ -93,14 +93,13 @@ class HipCoffeeParticleSystem:
         self.ctx.macroscopic(macro=True)
 
     def coupled_step(self, solver=None, relaxation_factor: Optional[float] = None, nsteps: int = 1) -> None:
-        """couple -> fused LBM step (writes u) -> advance, nsteps times."""
+        """couple -> fused LBM step (writes rho/u) -> advance, nsteps times, in
+        one native loop (pol_coupled_step)."""
         if relaxation_factor is not None and abs(relaxation_factor - self.desc.relax_alpha) > 0:
             self.desc.relax_alpha = float(relaxation_factor)
             self._bind()
-        for _ in range(nsteps):
-            self.couple()
-            self.ctx.step(1, L.POL_WANT_MACRO)
-            self.advance()
+        st = self.lib.pol_coupled_step(self.ctx.handle, int(nsteps), 0, None, L.stream_handle())
+        L.check(st, self.ctx.handle, "pol_coupled_step")
         if solver is not None and hasattr(solver, "_macro_valid"):
             solver._step_count += nsteps
             solver._macro_valid = False
