@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   const uint32_t kind = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;
   const bool active = inside && kind != POL_KIND_SOLID;
 
-  double part[4] = {0.0, 0.0, 0.0, 0.0};  // dJ/du_in (3), J
+  R part[4] = {R(0), R(0), R(0), R(0)};  // this cell's dJ/du_in (3) and J; summed in FP64
   const bool px = a.options & POL_PERIODIC_X, py = a.options & POL_PERIODIC_Y, pz = a.options & POL_PERIODIC_Z;
   const int nx = (int)a.nx, ny = (int)a.ny, nz = (int)a.nz, nxy = (int)a.nxy;
   const int oxm = x == 0 ? (px ? nx - 1 : 0) : -1;
@@ -199,9 +199,9 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       const R ri = (R)a.inlet_rho, ux = (R)a.inlet_u[0], uy = (R)a.inlet_u[1], uz = (R)a.inlet_u[2];
       R gx, gy, gz;
       eq_grad_u<R>(M, ri, ux, uy, uz, gx, gy, gz);
-      part[0] = (double)gx;
-      part[1] = (double)gy;
-      part[2] = (double)gz;
+      part[0] = gx;
+      part[1] = gy;
+      part[2] = gz;
       // nothing gathered: mu at inlet cells is never read (left zero)
     } else {
       const S* __restrict__ fin = (const S*)a.fin;
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         hz = az * inv_rho;
         C = -(hx * vx + hy * vy + hz * vz);
         if (A.objective) hz -= R(1);
-        if (A.objective) part[3] = -(double)(mz + R(0.5) * F0z);
+        if (A.objective) part[3] = -(mz + R(0.5) * F0z);
       } else {
         const bool porous = fl & POL_FLAG_POROUS;
         const bool les = (a.options & POL_LES) && !(fl & POL_FLAG_NO_LES) && !porous;
@@ -448,7 +448,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    const double r = wave_sum(part[s]);
+    const double r = wave_sum((double)part[s]);
     if (lane == 0) sh[wid][s] = r;
   }
   __syncthreads();
