@@ -141,3 +141,19 @@ def test_struct_layouts_match_header(tmp_path):
         seen[tname].add(field)
     for tname, ct in _CTYPES.items():
         assert seen[tname] == {f[0] for f in ct._fields_}, tname
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No CPU fallback: without the HIP library the product path raises."""
+    import os
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r); import __graft_entry__ as g; g.load_package(); "
+            "from pourover_amd import _lib\n"
+            "try:\n    _lib.load()\nexcept ImportError as e:\n    print('raised', e)\n    sys.exit(0)\n"
+            "sys.exit(3)") % str(ROOT)
+    env = dict(os.environ, POUROVER_HIP_LIB=str(tmp_path / "absent.so"))
+    p = subprocess.run([sys.executable, "-c", code], env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "raised" in p.stdout

@@ -112,6 +112,39 @@ def test_porous_terminal_velocity(oracle, K, CF, G):
     assert np.abs(u[:2]).max() < 1e-15
 
 
+def test_forchheimer_force_in_filter_cells_only(oracle):
+    """tests/test_forchheimer.py:31-77 of the reference: with u = (0.02, 0,
+    -0.015) everywhere, the Darcy-Forchheimer force is non-zero in the filter
+    zone, opposes u there, and vanishes elsewhere.  Here: one step from the
+    uniform equilibrium in a periodic box with a porous z-slab; the momentum
+    each cell gains in the step is the force."""
+    n, tau, K, CF = 6, 0.6, 1e-2, 0.55
+    u0 = np.array([0.02, 0.0, -0.015])
+    flags = _flags_box((n, n, n))
+    flags[2:4] |= np.uint32(L.POL_FLAG_POROUS | L.POL_FLAG_NO_LES)
+    d = oracle.make_desc(n, n, n, tau=tau, options=PER | L.POL_USE_FLAGS, darcy_K=K, forch_CF=CF)
+    rho0 = np.ones((n, n, n))
+    u = np.broadcast_to(u0[:, None, None, None], (3, n, n, n)).copy()
+    f0 = oracle.init_equilibrium(n, n, n, rho0, u)
+    f1, _, _, _ = oracle.step(d, f0, flags)
+    E3 = np.asarray(E, dtype=np.float64)
+    m0 = np.einsum("qa,qzyx->azyx", E3, f0)
+    m1 = np.einsum("qa,qzyx->azyx", E3, f1)
+    # periodic box, uniform state: streaming moves equal values, so the
+    # change of sum f e at a cell is the force applied at that cell's source
+    dm = m1 - m0
+    por = np.zeros((n, n, n), bool)
+    por[2:4] = True
+    inside = dm[:, por]
+    assert np.all(np.linalg.norm(inside, axis=0) > 1e-6)
+    assert np.all(np.einsum("a,ai->i", u0, inside) < 0)            # opposes u
+    # cells whose whole neighbourhood is outside the slab feel nothing
+    far = np.zeros((n, n, n), bool)
+    far[0] = True
+    far[5] = True
+    assert np.abs(dm[:, far]).max() < 1e-15
+
+
 def _shear_state(rho, tau_t, gamma):
     """feq(rho, 0) + non-equilibrium part with stress Pi = -2 rho c_s^2 tau_t S,
     S_xz = S_zx = gamma / 2 (Hermite projection w/(2 c_s^4) (e e - c_s^2 I):Pi)."""
