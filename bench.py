@@ -297,6 +297,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only (no secondary lines, no CPU baseline)")
     ap.add_argument("--precision", default="f32")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="only the two-way coupled 224^3 step with 50 000 particles (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
@@ -312,6 +314,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     prec = args.precision
+    if args.coupled_only:
+        log("bench: coupled 224^3 %s + 50k particles" % prec)
+        print(json.dumps({"coupled_224_50k_particles": bench_coupled(prec, args.steps, args.warmup)}), flush=True)
+        return
     if args.calibrate_copy:
         out = copy_calibration()
         log("bench: copy calibration %s" % json.dumps(out))

@@ -6,8 +6,9 @@
 // computes moments, collides (BGK + Guo forcing + Smagorinsky + Darcy-
 // Forchheimer, see cell_model.h) and writes its 19 post-collision values to
 // the other buffer at its own index.  Every population is read once and
-// written once: 152 B per fluid cell update in FP32, 76 B in 16-bit storage;
-// solid cells are never read or written.
+// written once: 152 B per fluid cell update in FP32, 76 B in 16-bit storage.
+// Solid cells get no threads, except the solid tails of launched row chunks,
+// which store zero deviations so that written lines are whole.
 //
 // Replaces (reference): cpu_backend.py:79-166 (collision + pull streaming),
 // legacy/lbm_solver.py:488-654 (moments, collision, push streaming with
