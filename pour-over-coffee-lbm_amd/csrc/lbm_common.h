@@ -200,22 +200,27 @@ __device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t
 
 // Work item -> cell for the compact layout: lane (item & (2^cshift - 1)) of
 // the chunk described by records[item >> cshift] (POL_RECORD_WORDS words:
-// dense first cell, the nine neighbour-row bases, the own row's run).  rb[i]
-// is the slot base of row (y + i%3 - 1, z + i/3 - 1); in_run tells whether x
-// lies in the own row's stored run.
+// dense first cell, the nine neighbour-row bases, the own row's stored run).
+// rb[i] is the slot base of row (y + i%3 - 1, z + i/3 - 1).  sx = x0 + lane
+// is the lane's x counted from its chunk's row without wrapping (a stored run
+// may extend past the row end as padding), and in_run tells whether sx lies
+// in the own row's stored run, i.e. whether the lane owns slot rb[4] + sx.
 __device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ records,
                                    uint32_t cshift, uint32_t N, const FastDiv& dnx, uint32_t nx, uint32_t& cell,
-                                   uint32_t& yz, uint32_t& x, int (&rb)[9], bool& in_run) {
-  bool ok = item < n_items;
+                                   uint32_t& yz, uint32_t& x, int (&rb)[9], int& sx, bool& in_run) {
+  const bool listed = item < n_items;
+  bool ok = listed;
   const uint32_t lane = item & ((1u << cshift) - 1u);
   uint32_t w[12] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-  if (ok) {
+  if (listed) {
     const uint4* r = (const uint4*)(records + (size_t)(item >> cshift) * POL_RECORD_WORDS);
     const uint4 a = r[0], b = r[1], c = r[2];
     w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
     w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
     w[8] = c.x; w[9] = c.y; w[10] = c.z; w[11] = c.w;
   }
+  const uint32_t yz0 = fdiv(w[0], dnx);
+  sx = (int)(w[0] - yz0 * nx + lane);
   uint32_t cc = w[0] + lane;
   ok = ok && cc < N;
   if (!ok) cc = 0;
@@ -224,7 +229,7 @@ __device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32
   if (x < lane) ok = false;  // wrapped into the next row
 #pragma unroll
   for (int i = 0; i < 9; ++i) rb[i] = (int)w[1 + i];
-  in_run = ok && x >= (w[10] & 0xffffu) && x < (w[10] >> 16);
+  in_run = listed && sx >= (int)(w[10] & 0xffffu) && sx < (int)(w[10] >> 16);
   cell = ok ? cc : 0u;
   if (!ok) {
     yz = 0;

@@ -76,13 +76,17 @@ __global__ void __launch_bounds__(BLOCK) k_init_equilibrium_sparse(const StepArg
   if (lblk >= a.nblk) return;
   uint32_t cell, yz, x;
   int rb[9];
+  int sx = 0;
   bool in_run = false;
-  item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz, x, rb, in_run);
+  const bool ok = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz,
+                              x, rb, sx, in_run);
   if (!in_run) return;
   const uint32_t N = a.N, P = a.P;
-  const uint32_t own = slot_clamp(rb[4] + (int)x, P);
-  const R r = rho ? rho[cell] : R(1);
-  const R ux = u ? u[cell] : R(0), uy = u ? u[(size_t)N + cell] : R(0), uz = u ? u[2 * (size_t)N + cell] : R(0);
+  const uint32_t own = slot_clamp(rb[4] + sx, P);
+  // padding slots past the row end take the rest state
+  const R r = (rho && ok) ? rho[cell] : R(1);
+  const R ux = (u && ok) ? u[cell] : R(0), uy = (u && ok) ? u[(size_t)N + cell] : R(0),
+          uz = (u && ok) ? u[2 * (size_t)N + cell] : R(0);
   const R usq = ux * ux + uy * uy + uz * uz;
   S* __restrict__ f = (S*)a.fout;
   static_for<0, Q>([&](auto qc) {

@@ -18,11 +18,12 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   if (lblk >= a.nblk) return;  // whole padding block exits (uniform)
   uint32_t cell, yz, x;
   int rb[9];          // compact layout: slot bases of the nine neighbour rows
+  int sx = 0;         // compact layout: the lane's unwrapped x (its slot is rb[4] + sx)
   bool in_run = false;
   bool inside;
   if constexpr (SPARSE)
     inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz, x,
-                         rb, in_run);
+                         rb, sx, in_run);
   else
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
   const uint32_t z = fdiv(yz, a.dny);
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   const uint32_t PS = a.P;  // population plane stride
   // population slot of this cell: the dense index, or its compact-layout slot
   uint32_t own = cell;
-  if constexpr (SPARSE) own = slot_clamp(rb[4] + (int)x, PS);
+  if constexpr (SPARSE) own = slot_clamp(rb[4] + sx, PS);
 
   R f[Q];  // scaled deviations 36 (f - w)
   CellOut<R> o{R(0), R(0), R(0), R(0)};

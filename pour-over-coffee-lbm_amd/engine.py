@@ -17,15 +17,21 @@ from .layout import build_compact_layout
 
 POL_CHUNK = L.POL_CHUNK
 WRITE_GRANULE = 128  # bytes: HBM writes narrower than this line cost a read-modify-write
-# Population layout and chunk size by storage, from the 224^3 V60 step:
-#   dense layout, FP32 16 / 32 / 64-cell chunks: 0.174 / 0.145 / 0.159 ms
-#   compact layout, FP32 16 / 32 / 64-cell chunks: 0.166 / 0.154 / 0.143 ms
-#     (15 % fewer HBM bytes, but unaligned runs cost as much in requests)
-#   dense layout, 16-bit 32-cell chunks: 0.098 ms
-#   compact layout, 16-bit 16 / 32 / 64-cell chunks: 0.091 / 0.093 / 0.100 ms
-COMPACT_DEFAULT = {"f32": False, "f16": True, "bf16": True, "f64": False}
+# Population layout, chunk size and run alignment by storage, from the
+# 224^3 V60 step (ms per step):
+#   dense layout, 16 / 32 / 64-cell chunks: FP32 0.174 / 0.144 / 0.159,
+#     16-bit 0.114 / 0.098 / 0.110, FP64 0.255 (32)
+#   compact, runs packed (align 1): FP32 0.166 / 0.154 / 0.141,
+#     16-bit 0.091 / 0.093 / 0.100, FP64 0.294 (16)
+#   compact, (chunk, align) = (8, 8) / (16, 16) / (32, 32):
+#     FP32 0.125 / 0.124 / 0.127, FP16 0.081 / 0.082 / 0.087,
+#     BF16 - / 0.090 / -, FP64 0.239 / 0.249 / -
+# Aligned runs write whole aligned segments and leave 2-6 % padding where
+# the dense chunks leave 23 % solid lanes.
+COMPACT_DEFAULT = {"f32": True, "f16": True, "bf16": True, "f64": True}
 CHUNK_CELLS = {"f32": 32, "f16": 32, "bf16": 32, "f64": 16}
-CHUNK_CELLS_COMPACT = {"f32": 64, "f16": 16, "bf16": 16, "f64": 32}
+CHUNK_CELLS_COMPACT = {"f32": 16, "f16": 8, "bf16": 16, "f64": 8}
+COMPACT_ALIGN = {"f32": 16, "f16": 8, "bf16": 16, "f64": 8}
 _STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 
 
@@ -116,9 +122,10 @@ class HipContext:
     """One pol_ctx with its device buffers."""
 
     def __init__(self, cfg: SolverConfig, flags: Optional[np.ndarray] = None, chunk_cells: Optional[int] = None,
-                 compact: Optional[bool] = None):
-        """compact: store populations in the row-run layout (layout.py); by
-        default on for 16-bit storage whenever cell flags are given."""
+                 compact: Optional[bool] = None, align: Optional[int] = None):
+        """compact: store populations in the row-run layout (layout.py), on
+        by default whenever cell flags are given; chunk_cells / align override
+        the per-storage launch chunk and run alignment."""
         if not torch.cuda.is_available():
             raise RuntimeError("HipContext needs a visible HIP device (torch.cuda.is_available() is False)")
         self.lib = L.load()
@@ -134,6 +141,7 @@ class HipContext:
         self.compact = (flags is not None and COMPACT_DEFAULT[cfg.precision]) if compact is None else bool(compact)
         table = CHUNK_CELLS_COMPACT if self.compact else CHUNK_CELLS
         self.chunk_cells = int(chunk_cells or table[cfg.precision])
+        self.align = int(align or COMPACT_ALIGN[cfg.precision]) if self.compact else 1
         if self.compact and flags is None:
             raise ValueError("the compact layout needs cell flags")
         self._set_layout(flags)
@@ -176,7 +184,8 @@ class HipContext:
         flags are given) a dense chunk list."""
         cfg = self.config
         if self.compact:
-            self.layout = build_compact_layout(flags, tuple(bool(p) for p in cfg.periodic), self.chunk_cells)
+            self.layout = build_compact_layout(flags, tuple(bool(p) for p in cfg.periodic), self.chunk_cells,
+                                               self.align)
             self.records = _as_int32_view(self.layout.records.reshape(-1)).to(self.device)
             self.chunks = None
             self.plane = self.layout.stored_cells
@@ -193,8 +202,9 @@ class HipContext:
         if self.layout is None:
             return t.view((19,) + self.shape)
         idx = torch.from_numpy(self.layout.dense_index).to(self.device)
+        slots = torch.from_numpy(self.layout.run_slots).to(self.device)
         out = torch.zeros((19, self.cells), dtype=t.dtype, device=self.device)
-        out[:, idx] = t[:, :self.layout.used_cells]
+        out[:, idx] = t[:, slots]
         return out.view((19,) + self.shape)
 
     def load_dense_populations(self, dense: torch.Tensor, t: Optional[torch.Tensor] = None) -> None:
@@ -205,8 +215,9 @@ class HipContext:
             t.copy_(d)
             return
         idx = torch.from_numpy(self.layout.dense_index).to(self.device)
+        slots = torch.from_numpy(self.layout.run_slots).to(self.device)
         t.zero_()
-        t[:, :self.layout.used_cells] = d[:, idx]
+        t[:, slots] = d[:, idx]
 
     def _chunk_list(self, flags: Optional[np.ndarray]) -> Optional[torch.Tensor]:
         """Launch list: first cells of the chunk_cells-cell row chunks holding

@@ -26,7 +26,10 @@ RECORD_WORDS uint32 per 32-cell row chunk that meets a run,
     [11]    0
 so a thread finds every neighbour slot from its record without further
 lookups: source of link q = word[1 + (1 - e_y) + 3 (1 - e_z)] + x - e_x
-(x wrapped or clamped like the dense gather).
+(x wrapped or clamped like the dense gather).  With align > 1 each row's
+storage starts on an aligned slot and is padded to a multiple of align, and
+the chunks are run pieces starting at the run's first cell (see
+build_compact_layout).
 """
 from __future__ import annotations
 
@@ -46,15 +49,26 @@ class CompactLayout:
     ny: int
     nz: int
     stored_cells: int        # M: plane stride in slots
-    used_cells: int          # slots holding a run cell (the rest is padding)
+    used_cells: int          # slots of the stored runs (padding included)
     row_base: np.ndarray     # int64 [nz*ny]
-    row_run: np.ndarray      # uint32 [nz*ny]: x_begin | x_end << 16 (0 = empty row)
+    row_run: np.ndarray      # uint32 [nz*ny]: x_begin | x_end << 16 (0 = empty row); x_end may pass nx (padding)
     records: np.ndarray      # uint32 [n_chunks, RECORD_WORDS]
-    dense_index: np.ndarray  # int64 [used_cells]: dense cell of slot s
+    slot_dense: np.ndarray   # int64 [used_cells]: dense cell of slot s, -1 for padding
+    align: int = 1           # slots a row's storage is aligned and padded to
 
     @property
     def n_chunks(self) -> int:
         return int(self.records.shape[0])
+
+    @property
+    def dense_index(self) -> np.ndarray:
+        """Dense cells of the run slots in slot order (padding left out)."""
+        return self.slot_dense[self.slot_dense >= 0]
+
+    @property
+    def run_slots(self) -> np.ndarray:
+        """Slots that hold a run cell (the slots dense_index maps)."""
+        return np.nonzero(self.slot_dense >= 0)[0]
 
     def slot_of(self, x, y, z):
         """Slot of dense cell (x, y, z) (valid for cells inside their row's run)."""
@@ -64,7 +78,7 @@ class CompactLayout:
         """[C, M] stored planes -> [C, nz, ny, nx] (cells outside every run = fill)."""
         c = pop.shape[0]
         out = np.full((c, self.nz * self.ny * self.nx), fill, dtype=pop.dtype)
-        out[:, self.dense_index] = pop[:, :self.used_cells]
+        out[:, self.dense_index] = pop[:, self.run_slots]
         return out.reshape(c, self.nz, self.ny, self.nx)
 
     def from_dense(self, dense: np.ndarray) -> np.ndarray:
@@ -72,15 +86,26 @@ class CompactLayout:
         c = dense.shape[0]
         flat = dense.reshape(c, -1)
         out = np.zeros((c, self.stored_cells), dtype=dense.dtype)
-        out[:, :self.used_cells] = flat[:, self.dense_index]
+        out[:, self.run_slots] = flat[:, self.dense_index]
         return out
 
 
-def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chunk_cells: int = 32) -> CompactLayout:
-    """Row runs, slot map and chunk records of a cell-flag array [nz, ny, nx]."""
+def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chunk_cells: int = 32,
+                         align: int = 1) -> CompactLayout:
+    """Row runs, slot map and chunk records of a cell-flag array [nz, ny, nx].
+
+    align = 1 packs the runs back to back and launches the chunk_cells-cell
+    row chunks (x0 a multiple of chunk_cells) that meet a run.  align > 1
+    (dividing chunk_cells) starts every row's storage on a multiple of align
+    slots, pads it to a multiple of align, and launches run pieces of
+    chunk_cells cells from the run's first cell, so every launched chunk
+    writes whole aligned slot segments (padding slots take zero deviations).
+    """
     flags = np.asarray(flags, dtype=np.uint32)
     nz, ny, nx = flags.shape
-    if nx >= 1 << 16:
+    if chunk_cells % align:
+        raise ValueError("align must divide chunk_cells")
+    if nx + align >= 1 << 16:
         raise ValueError("the compact layout packs x in 16 bits (nx < 65536)")
     kind = (flags >> np.uint32(L.POL_KIND_SHIFT)) & np.uint32(L.POL_KIND_MASK)
     open_cell = (kind != L.POL_KIND_SOLID).reshape(nz * ny, nx)
@@ -88,29 +113,38 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
     xb = np.where(has, open_cell.argmax(axis=1), 0).astype(np.int64)
     xe = np.where(has, nx - open_cell[:, ::-1].argmax(axis=1), 0).astype(np.int64)
     length = xe - xb
-    start = np.concatenate([[0], np.cumsum(length)[:-1]]).astype(np.int64)
-    used = int(length.sum())
+    padded = -(-length // align) * align
+    start = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
+    used = int(padded.sum())
     if used == 0:
         raise ValueError("no non-solid cell: nothing to store")
     M = -(-used // PLANE_ALIGN) * PLANE_ALIGN
     if M >= 1 << 31:
         raise ValueError("too many stored cells for 32-bit slots")
     row_base = np.where(has, start - xb, 0).astype(np.int64)
-    row_run = (xb | (xe << 16)).astype(np.uint32)
+    row_run = (xb | ((xb + padded) << 16)).astype(np.uint32)
     row_run[~has] = 0
-    # dense index of every slot
-    rows = np.repeat(np.arange(nz * ny, dtype=np.int64), length)
-    offs = np.arange(used, dtype=np.int64) - np.repeat(start, length)
-    dense_index = rows * nx + np.repeat(xb, length) + offs
-    # chunk list: the chunk_cells-cell row chunks that meet a row's run (so
-    # every stored slot is written each step, solid run cells with zeros)
-    nch = -(-nx // chunk_cells)
-    c0 = np.arange(nch, dtype=np.int64) * chunk_cells
-    live = (c0[None, :] < xe[:, None]) & (c0[None, :] + chunk_cells > xb[:, None]) & has[:, None]
-    live = live.reshape(nz, ny, nch)
-    zz, yy, cc = np.nonzero(live)
-    rec = np.zeros((zz.size, RECORD_WORDS), dtype=np.int64)
-    rec[:, 0] = (zz * ny + yy) * nx + cc * chunk_cells
+    # dense cell of every slot (-1 for padding)
+    rows = np.repeat(np.arange(nz * ny, dtype=np.int64), padded)
+    offs = np.arange(used, dtype=np.int64) - np.repeat(start, padded)
+    slot_dense = np.where(offs < np.repeat(length, padded), rows * nx + np.repeat(xb, padded) + offs, -1)
+    if align == 1:
+        # chunk_cells-cell row chunks (x0 a multiple of chunk_cells) that meet a
+        # row's run (so every stored slot is written each step)
+        nch = -(-nx // chunk_cells)
+        c0 = np.arange(nch, dtype=np.int64) * chunk_cells
+        live = (c0[None, :] < xe[:, None]) & (c0[None, :] + chunk_cells > xb[:, None]) & has[:, None]
+        rr, cc = np.nonzero(live)
+        first = rr * nx + cc * chunk_cells
+    else:
+        # run pieces of chunk_cells cells from each run's first cell
+        pieces = -(-padded // chunk_cells)
+        rr = np.repeat(np.arange(nz * ny, dtype=np.int64), pieces)
+        k = np.arange(int(pieces.sum()), dtype=np.int64) - np.repeat(np.cumsum(pieces) - pieces, pieces)
+        first = rr * nx + xb[rr] + k * chunk_cells
+    zz, yy = rr // ny, rr % ny
+    rec = np.zeros((rr.size, RECORD_WORDS), dtype=np.int64)
+    rec[:, 0] = first
     py, pz = bool(periodic[1]), bool(periodic[2])
     for b in range(3):
         z2 = zz + (b - 1)
@@ -119,6 +153,6 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
             y2 = yy + (a - 1)
             y2 = np.where((y2 < 0) | (y2 >= ny), (y2 % ny) if py else yy, y2)
             rec[:, 1 + a + 3 * b] = row_base[z2 * ny + y2]
-    rec[:, 10] = row_run[zz * ny + yy]
+    rec[:, 10] = row_run[rr]
     records = (rec & 0xFFFFFFFF).astype(np.uint32)  # row bases as two's-complement int32
-    return CompactLayout(nx, ny, nz, M, used, row_base, row_run, records, dense_index)
+    return CompactLayout(nx, ny, nz, M, used, row_base, row_run, records, slot_dense, align)

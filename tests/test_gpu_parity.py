@@ -139,15 +139,16 @@ def test_step_from_and_current_buffer(hip):
 
 @pytest.mark.parametrize("precision", ["f64", "f32", "f16"])
 def test_compact_layout_matches_dense(hip, precision):
-    """Row-run population layout vs the dense layout on the scaled cup: the
-    same arithmetic on the same values, so moments and populations agree bit
-    for bit after 200 steps (chunk sizes differ too; only the order of the
-    FP64 block sums of the reductions changes)."""
+    """Row-run population layout (aligned runs, the default, and packed runs)
+    vs the dense layout on the scaled cup: the same arithmetic on the same
+    values, so moments and populations agree bit for bit after 200 steps
+    (chunk sizes differ too; only the order of the FP64 block sums of the
+    reductions changes)."""
     wl = WL.v60_config(48, precision)
     flags = wl.geometry.flags
     out = []
-    for compact in (False, True):
-        ctx = HipContext(wl.config, flags, compact=compact)
+    for compact, ch, al in ((False, None, None), (True, None, None), (True, 32, 1)):
+        ctx = HipContext(wl.config, flags, compact=compact, chunk_cells=ch, align=al)
         ctx.init_equilibrium()
         ctx.step(200)
         red = ctx.macroscopic(macro=True, reduce=True)
@@ -155,10 +156,11 @@ def test_compact_layout_matches_dense(hip, precision):
         kind = wl.geometry.kind
         pops = ctx.dense_populations().cpu().numpy()[:, kind != 1]
         out.append((ctx.rho.cpu().numpy(), ctx.u.cpu().numpy(), red, pops, ctx.plane))
-    (rd, ud, redd, pd, Nd), (rc, uc, redc, pc, Nc) = out
-    assert Nc < Nd
-    assert np.array_equal(rd, rc) and np.array_equal(ud, uc)
-    assert np.array_equal(pd, pc)
-    # FP64 block sums are added in launch order, which the chunk size changes
-    # (sums of near-cancelling momenta compare on the mass scale)
-    assert np.allclose(redd, redc, rtol=1e-12, atol=1e-12 * abs(redd[0]))
+    rd, ud, redd, pd, Nd = out[0]
+    for rc, uc, redc, pc, Nc in out[1:]:
+        assert Nc < Nd
+        assert np.array_equal(rd, rc) and np.array_equal(ud, uc)
+        assert np.array_equal(pd, pc)
+        # FP64 block sums are added in launch order, which the chunk size changes
+        # (sums of near-cancelling momenta compare on the mass scale)
+        assert np.allclose(redd, redc, rtol=1e-12, atol=1e-12 * abs(redd[0]))

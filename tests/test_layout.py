@@ -12,17 +12,27 @@ from pourover_amd.lattice import E, OPP, Q
 from pourover_amd.layout import RECORD_WORDS, build_compact_layout
 
 
-def _lanes(lay, chunk_cells=32):
-    """(record row, x, y, z) of every lane of every chunk (lanes past the row end dropped)."""
+def _lanes(lay, chunk_cells=32, with_padding=False):
+    """(record row, x, y, z) of every lane of every chunk; lanes past the row
+    end dropped (the kernel's wrap test), or with with_padding also returned
+    with their unwrapped x (record row, sx, y, z, valid)."""
     rec = lay.records
     base = rec[:, 0].astype(np.int64)
     lane = np.arange(chunk_cells, dtype=np.int64)
     cell = base[:, None] + lane[None, :]
     yz = cell // lay.nx
     x = cell % lay.nx
-    ok = (x >= lane[None, :]) & (cell < lay.nx * lay.ny * lay.nz)  # the kernel's wrap test
+    ok = (x >= lane[None, :]) & (cell < lay.nx * lay.ny * lay.nz)
     r = np.broadcast_to(np.arange(rec.shape[0])[:, None], cell.shape)
+    if with_padding:
+        yz0 = base // lay.nx
+        sx = (base - yz0 * lay.nx)[:, None] + lane[None, :]
+        return (r.ravel(), sx.ravel(), np.broadcast_to((yz0 % lay.ny)[:, None], cell.shape).ravel(),
+                np.broadcast_to((yz0 // lay.ny)[:, None], cell.shape).ravel(), ok.ravel())
     return r[ok], x[ok], (yz % lay.ny)[ok], (yz // lay.ny)[ok]
+
+
+LAYOUTS = [(1, 32), (32, 32), (16, 16), (32, 64), (64, 64)]
 
 
 def _rb(lay, r, a, b):
@@ -49,13 +59,14 @@ def _unpack(g):
     return g.kind, g.flags, g.rev_flags, (False, False, False)
 
 
+@pytest.mark.parametrize("align,chunk", LAYOUTS)
 @pytest.mark.parametrize("name", ["v60_48", "periodic_xy"])
-def test_gather_slots_match_dense_sources(name):
+def test_gather_slots_match_dense_sources(name, align, chunk):
     kind, flags, rev, per = _unpack(_geometries()[name])
     nz, ny, nx = kind.shape
-    lay = build_compact_layout(flags, per)
+    lay = build_compact_layout(flags, per, chunk_cells=chunk, align=align)
     assert lay.records.shape[1] == RECORD_WORDS
-    r, x, y, z = _lanes(lay)
+    r, x, y, z = _lanes(lay, chunk)
     k = kind[z, y, x]
     run = lay.records[r, 10]
     inrun = (x >= (run & 0xFFFF)) & (x < (run >> 16))
@@ -88,12 +99,13 @@ def test_gather_slots_match_dense_sources(name):
         assert np.array_equal(slot[m], lay.slot_of(xs[m], ys[m], zs[m])), q
 
 
+@pytest.mark.parametrize("align,chunk", LAYOUTS)
 @pytest.mark.parametrize("name", ["v60_48", "periodic_xy"])
-def test_reverse_gather_slots(name):
+def test_reverse_gather_slots(name, align, chunk):
     kind, flags, rev, per = _unpack(_geometries()[name])
     nz, ny, nx = kind.shape
-    lay = build_compact_layout(flags, per)
-    r, x, y, z = _lanes(lay)
+    lay = build_compact_layout(flags, per, chunk_cells=chunk, align=align)
+    r, x, y, z = _lanes(lay, chunk)
     act = kind[z, y, x] != L.POL_KIND_SOLID
     rf = rev[z, y, x]
     for p in range(Q):
@@ -107,9 +119,10 @@ def test_reverse_gather_slots(name):
         assert np.array_equal(slot[has], lay.slot_of(xr[has], yr[has], zr[has])), p
 
 
-def test_dense_round_trip_and_every_slot_written():
+@pytest.mark.parametrize("align,chunk", LAYOUTS)
+def test_dense_round_trip_and_every_slot_written(align, chunk):
     g = G.build_v60(G.V60Params.scaled(48))
-    lay = build_compact_layout(g.flags)
+    lay = build_compact_layout(g.flags, chunk_cells=chunk, align=align)
     rng = np.random.default_rng(3)
     dense = rng.standard_normal((19,) + g.kind.shape)
     comp = lay.from_dense(dense)
@@ -117,13 +130,24 @@ def test_dense_round_trip_and_every_slot_written():
     back = lay.to_dense(comp, fill=np.nan)
     inrun = np.zeros(g.kind.size, bool)
     inrun[lay.dense_index] = True
+    assert inrun.sum() == (g.kind != L.POL_KIND_SOLID).sum()
     assert np.array_equal(back.reshape(19, -1)[:, inrun], dense.reshape(19, -1)[:, inrun])
     assert np.isnan(back.reshape(19, -1)[:, ~inrun]).all()
-    # every used slot is a lane inside its run (written each step)
-    r, x, y, z = _lanes(lay)
-    run = lay.records[r, 10]
-    inr = (x >= (run & 0xFFFF)) & (x < (run >> 16))
-    slots = np.sort(lay.slot_of(x[inr], y[inr], z[inr]))
+    # every used slot (padding included) is owned by exactly one lane
+    r, sx, y, z, ok = _lanes(lay, chunk, with_padding=True)
+    run = lay.records[r, 10].astype(np.int64)
+    inr = (sx >= (run & 0xFFFF)) & (sx < (run >> 16))
+    slots = np.sort(lay.row_base[z[inr] * lay.ny + y[inr]] + sx[inr])
     assert np.array_equal(slots, np.arange(lay.used_cells))
-    # fewer slots than the dense chunk lanes
-    assert lay.used_cells < 0.9 * lay.n_chunks * 32
+    # a padding lane is never a valid cell of the grid
+    pad = inr & (lay.slot_dense[np.clip(lay.row_base[z * lay.ny + y] + sx, 0, lay.used_cells - 1)] < 0)
+    assert not (pad & ok & (sx < lay.nx) & (g.kind.reshape(-1)[np.clip((z * lay.ny + y) * lay.nx + sx, 0, g.kind.size - 1)]
+                                            != L.POL_KIND_SOLID)).any()
+    if align > 1:
+        # every row's storage starts aligned and launched pieces write whole segments
+        starts = (lay.row_base + (lay.row_run & 0xFFFF))[lay.row_run != 0]
+        assert (starts % align == 0).all()
+        assert lay.used_cells % align == 0
+    if align == 1:
+        # packed runs: one slot per run cell
+        assert lay.used_cells == int(((lay.row_run >> 16) - (lay.row_run & 0xFFFF)).astype(np.int64).sum())
