@@ -299,6 +299,8 @@ def main():
     ap.add_argument("--precision", default="f32")
     ap.add_argument("--coupled-only", action="store_true",
                     help="only the two-way coupled 224^3 step with 50 000 particles (for profiling)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the 128^3 adjoint measurement (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
@@ -317,6 +319,10 @@ def main():
     if args.coupled_only:
         log("bench: coupled 224^3 %s + 50k particles" % prec)
         print(json.dumps({"coupled_224_50k_particles": bench_coupled(prec, args.steps, args.warmup)}), flush=True)
+        return
+    if args.adjoint_only:
+        log("bench: adjoint 128^3 f32")
+        print(json.dumps({"adjoint": bench_adjoint()}), flush=True)
         return
     if args.calibrate_copy:
         out = copy_calibration()
