@@ -139,9 +139,12 @@ typedef struct pol_buffers {
   /* Compact row-run population layout (NULL = dense [19][nz][ny][nx]).
    * Population q of cell (x, y, z) lives at f[q * stored_cells + slot],
    * slot = row_base(y, z) + x, for the cells of the row's stored run [x_begin,
-   * x_end) (first to last non-solid cell of the row); rows follow one another
-   * in (z, y) order.  Launches then use one record of POL_RECORD_WORDS uint32
-   * per chunk_cells-cell row chunk meeting a run:
+   * x_end) (first to last non-solid cell of the row, possibly padded); rows
+   * follow one another in (z, y) order.  Slots 0, 1, stored_cells - 2 and
+   * stored_cells - 1 hold no cell (guards: the kernels clamp a row offset
+   * into [1, stored_cells - 2] and reach x +- 1 by constant offsets).
+   * Launches then use one record of POL_RECORD_WORDS uint32 per
+   * chunk_cells-cell row piece meeting a run:
    *   [0] dense index of the chunk's first cell, [1 + a + 3 b] row_base of
    *   row (y + a - 1, z + b - 1) (a, b in 0..2; a row past a non-periodic
    *   face is the chunk's own row, periodic faces wrap), [10] own row's run
@@ -150,8 +153,12 @@ typedef struct pol_buffers {
    * deviations).  active_chunks must be NULL with records. */
   const uint32_t *chunk_records;
   uint32_t stored_cells; /* plane stride of the compact layout */
-  uint32_t reserved2;
+  uint32_t layout_flags; /* POL_LAYOUT_* */
 } pol_buffers;
+/* No run of the compact layout holds a cell at x = 0 or x = nx - 1: the
+ * kernels then leave out the x-face wrap / clamp of the gather.  Set it only
+ * when true (otherwise x-face cells read wrong, in-bounds sources). */
+#define POL_LAYOUT_NO_X_FACE 1u
 #define POL_CHUNK 32 /* default chunk_cells */
 #define POL_RECORD_WORDS 12 /* uint32 per compact-layout chunk record */
 

@@ -21,6 +21,7 @@ POL_Q = 19
 POL_NRED = 12
 POL_CHUNK = 32  # cells per launch chunk
 POL_RECORD_WORDS = 12  # uint32 per compact-layout chunk record
+POL_LAYOUT_NO_X_FACE = 1  # pol_buffers.layout_flags: no run reaches x = 0 or nx - 1
 POL_OK, POL_ERR_INVALID, POL_ERR_HIP, POL_ERR_ALLOC, POL_ERR_NO_DEVICE, POL_ERR_STATE, POL_ERR_UNSUPPORTED = range(7)
 POL_F32, POL_F16, POL_BF16, POL_F64 = range(4)
 POL_PERIODIC_X = 0x1
@@ -65,7 +66,7 @@ class PolBuffers(C.Structure):
         ("perm", C.c_void_p), ("rho", C.c_void_p), ("u", C.c_void_p),
         ("active_chunks", C.c_void_p), ("n_chunks", C.c_uint32), ("chunk_cells", C.c_uint32),
         ("particle_drag", C.c_void_p),
-        ("chunk_records", C.c_void_p), ("stored_cells", C.c_uint32), ("reserved2", C.c_uint32),
+        ("chunk_records", C.c_void_p), ("stored_cells", C.c_uint32), ("layout_flags", C.c_uint32),
     ]
 
 

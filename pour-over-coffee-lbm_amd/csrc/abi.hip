@@ -155,8 +155,10 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
     return fail(c, POL_ERR_INVALID, "pol_bind: n_chunks out of range");
   if (listed && !(o & POL_USE_FLAGS))
     return fail(c, POL_ERR_INVALID, "pol_bind: a chunk list needs cell flags (solid lanes are masked by them)");
-  if (b->chunk_records && (b->stored_cells == 0 || b->stored_cells >= (1u << 31)))
-    return fail(c, POL_ERR_INVALID, "pol_bind: compact layout needs 0 < stored_cells < 2^31");
+  if (b->chunk_records && (b->stored_cells < 4 || b->stored_cells >= (1u << 31)))
+    return fail(c, POL_ERR_INVALID, "pol_bind: compact layout needs 4 <= stored_cells < 2^31");
+  if (b->layout_flags & ~POL_LAYOUT_NO_X_FACE)
+    return fail(c, POL_ERR_INVALID, "pol_bind: unknown layout_flags bits");
   if (b->chunk_records && ((uintptr_t)b->chunk_records & 15))
     return fail(c, POL_ERR_INVALID, "pol_bind: chunk_records must be 16-byte aligned");
   const Grid wg = listed ? make_grid((uint64_t)b->n_chunks << cshift) : c->grid;
@@ -232,6 +234,7 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.P = c->P;
   a.n_items = (c->b.active_chunks || c->b.chunk_records) ? c->b.n_chunks << c->cshift : c->N;
   a.cshift = c->cshift;
+  a.xface = (c->b.layout_flags & POL_LAYOUT_NO_X_FACE) ? 0u : 1u;
   // Streaming (non-temporal) stores once a step's population traffic (read +
   // write) exceeds the 256 MiB last-level cache: at 224^3 they cut the V60
   // step by 3 % (FP32) and 8 % (16-bit); at 128^3, where the step still
@@ -251,6 +254,17 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.inlet_rho = d.inlet_rho;
   for (int i = 0; i < 3; ++i) a.inlet_u[i] = d.inlet_u[i];
   a.outlet_rho = d.outlet_rho;
+  a.pf.tau0 = (float)a.tau0;
+  a.pf.omega0 = (float)(1.0 / a.tau0);
+  a.pf.nu0 = (float)a.nu0;
+  a.pf.les_c = (float)a.les_c;
+  a.pf.tau_max = (float)a.tau_max;
+  for (int i = 0; i < 3; ++i) a.pf.G[i] = (float)a.G[i];
+  a.pf.K = (float)a.K;
+  a.pf.CF = (float)a.CF;
+  a.pf.inlet_rho = (float)a.inlet_rho;
+  for (int i = 0; i < 3; ++i) a.pf.inlet_u[i] = (float)a.inlet_u[i];
+  a.pf.outlet_rho = (float)a.outlet_rho;
   return a;
 }
 

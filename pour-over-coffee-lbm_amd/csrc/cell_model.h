@@ -42,6 +42,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 template <typename R>
 struct Phys {
   R tau0;     // molecular relaxation time
+  R omega0;   // 1 / tau0 (read by the 16-bit-storage collision only)
   R nu0;      // (tau0 - 1/2)/3
   R les_c;    // 18 sqrt(2) C_s^2
   R tau_max;  // tau0 + 3 nu_sgs_max
@@ -256,6 +257,143 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
     }
     d[a] = keep * d[a] + sym + anti;
     d[b] = keep * d[b] + sym - anti;
+  });
+  return o;
+}
+
+// The same collision for 16-bit population storage.  Storing a deviation in
+// FP16 (BF16) rounds it to 2^-11 (2^-8) relative, three to five orders above
+// FP32 arithmetic rounding, so this form spends fewer instructions where the
+// FP32 form spends them on exactness:
+//   - 1 / (36 rho), 1 / tau, sqrt and 1/sqrt(K) use the hardware
+//     approximations (v_rcp_f32, v_sqrt_f32, v_rsq_f32: 1 ulp);
+//   - the non-equilibrium stress comes from the second moments of the pair
+//     sums, 36 Pi_neq = sum_q D_q e_q e_q - (12 rd I + 36 rho u u), which is
+//     the pairwise sum of collide_cell rewritten with the D3Q19 identities
+//     sum_q c_q e_a e_b = 12 delta_ab, sum_q c_q (e.u)^2 e_a e_b =
+//     4 (u.u delta_ab + 2 u_a u_b);
+//   - BGK + Guo per pair from five per-cell coefficients:
+//     sym = c (omega base - 3 cf u.F) + e.u (4.5 c omega rho e.u + 9 c cf e.F),
+//     anti = 3 c omega rho e.u + 3 c cf e.F.
+// Mass and momentum stay conserved to rounding (the identities above hold
+// term by term).  Physics, branches and outputs are those of collide_cell.
+template <bool FULL>
+__device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, bool porous, bool les, float K,
+                                                 float fx, float fy, float fz, float bx, float by, float bz, float B,
+                                                 const Phys<float>& P) {
+  using R = float;
+  CellOut<R> o;
+  if (FULL && kind == POL_KIND_INLET) {
+    o.rd = P.inlet_rho - R(1);
+    o.ux = P.inlet_u[0];
+    o.uy = P.inlet_u[1];
+    o.uz = P.inlet_u[2];
+    set_equilibrium<R>(d, o.rd, P.inlet_rho, o.ux, o.uy, o.uz);
+    return o;
+  }
+  R sp[9];  // pair sums d_a + d_b
+  R s36 = d[0], mx = R(0), my = R(0), mz = R(0);
+  static_for<0, 9>([&](auto pc) {
+    constexpr int p = decltype(pc)::value;
+    constexpr int a = PAIR_A(p);
+    constexpr int b = OPP(a);
+    sp[p] = d[a] + d[b];
+    s36 += sp[p];
+    const R A = d[a] - d[b];
+    eacc<EX(a)>(mx, A);
+    eacc<EY(a)>(my, A);
+    eacc<EZ(a)>(mz, A);
+  });
+  const R rd = s36 * R(1.0 / 36.0);
+  const R rho = R(1) + rd;
+  const R rho36 = R(36) + s36;
+  const R inv36 = rho36 > R(36e-12) ? __builtin_amdgcn_rcpf(rho36) : R(0);  // 1 / (36 rho)
+  R Fx = R(0), Fy = R(0), Fz = R(0);
+  if constexpr (FULL) {
+    Fx = P.G[0] + fx;
+    Fy = P.G[1] + fy;
+    Fz = P.G[2] + fz;
+  }
+  R ux, uy, uz;
+  if (FULL && (porous || B > R(0))) {
+    const R vx = (mx + R(18) * (Fx + bx)) * inv36;
+    const R vy = (my + R(18) * (Fy + by)) * inv36;
+    const R vz = (mz + R(18) * (Fz + bz)) * inv36;
+    R c0 = R(1) + R(18) * B * inv36, c1 = R(0), dk = R(0), fk = R(0);
+    if (porous) {
+      dk = P.nu0 * __builtin_amdgcn_rcpf(K);
+      fk = P.CF * __builtin_amdgcn_rsqf(K);
+      c0 += R(0.5) * dk;
+      c1 = R(0.5) * fk;
+    }
+    const R vmag = __builtin_amdgcn_sqrtf(vx * vx + vy * vy + vz * vz);
+    const R s = R(2) * __builtin_amdgcn_rcpf(c0 + __builtin_amdgcn_sqrtf(c0 * c0 + R(4) * c1 * vmag));
+    ux = vx * s;
+    uy = vy * s;
+    uz = vz * s;
+    const R umag = __builtin_amdgcn_sqrtf(ux * ux + uy * uy + uz * uz);
+    const R c = rho * (dk + fk * umag) + B;
+    Fx += bx - c * ux;
+    Fy += by - c * uy;
+    Fz += bz - c * uz;
+  } else {
+    ux = (mx + R(18) * Fx) * inv36;
+    uy = (my + R(18) * Fy) * inv36;
+    uz = (mz + R(18) * Fz) * inv36;
+  }
+  o.rd = rd;
+  o.ux = ux;
+  o.uy = uy;
+  o.uz = uz;
+  if (FULL && kind == POL_KIND_OUTLET) {
+    set_equilibrium<R>(d, P.outlet_rho - R(1), P.outlet_rho, ux, uy, uz);
+    return o;
+  }
+  const R usq = ux * ux + uy * uy + uz * uz;
+  const R base = rd - R(1.5) * rho * usq;
+  R omega = P.omega0;
+  if (FULL && les && !porous) {
+    R sxx = R(0), syy = R(0), szz = R(0), sxy = R(0), sxz = R(0), syz = R(0);
+    static_for<0, 9>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      constexpr int a = PAIR_A(p);
+      if constexpr (EX(a) != 0) sxx += sp[p];
+      if constexpr (EY(a) != 0) syy += sp[p];
+      if constexpr (EZ(a) != 0) szz += sp[p];
+      eacc<EX(a) * EY(a)>(sxy, sp[p]);
+      eacc<EX(a) * EZ(a)>(sxz, sp[p]);
+      eacc<EY(a) * EZ(a)>(syz, sp[p]);
+    });
+    const R r36 = R(36) * rho, i12 = R(12) * rd;
+    const R pxx = sxx - (i12 + r36 * ux * ux), pyy = syy - (i12 + r36 * uy * uy), pzz = szz - (i12 + r36 * uz * uz);
+    const R pxy = sxy - r36 * ux * uy, pxz = sxz - r36 * ux * uz, pyz = syz - r36 * uy * uz;
+    const R q36 = __builtin_amdgcn_sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+    R tau = R(0.5) * (P.tau0 + __builtin_amdgcn_sqrtf(P.tau0 * P.tau0 + P.les_c * q36 * inv36));
+    tau = tau < P.tau_max ? tau : P.tau_max;
+    omega = __builtin_amdgcn_rcpf(tau);
+  }
+  const R keep = R(1) - omega;
+  const R cf = FULL ? (R(1) - R(0.5) * omega) : R(0);
+  const R uF = FULL ? (ux * Fx + uy * Fy + uz * Fz) : R(0);
+  // per-cell coefficients for c = 1; c = 2 doubles them exactly
+  const R S0 = omega * base - R(3) * cf * uF;
+  const R S1 = R(4.5) * omega * rho, S2 = R(9) * cf;
+  const R A1 = R(3) * omega * rho, A2 = R(3) * cf;
+  d[0] = keep * d[0] + R(12) * S0;
+  static_for<0, 9>([&](auto pc) {
+    constexpr int a = PAIR_A(decltype(pc)::value);
+    constexpr int b = OPP(a);
+    constexpr R c = CW<R>(a);
+    const R eu = edot<EX(a), EY(a), EZ(a)>(ux, uy, uz);
+    R t = (c * S1) * eu, anti = (c * A1) * eu;
+    if constexpr (FULL) {
+      const R eF = edot<EX(a), EY(a), EZ(a)>(Fx, Fy, Fz);
+      t += (c * S2) * eF;
+      anti += (c * A2) * eF;
+    }
+    const R sym = eu * t + c * S0;
+    d[a] = keep * d[a] + (sym + anti);
+    d[b] = keep * d[b] + (sym - anti);
   });
   return o;
 }

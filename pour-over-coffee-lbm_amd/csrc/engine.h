@@ -9,6 +9,15 @@ namespace pol {
 
 // Everything one stream-collide launch needs; physics kept in FP64 and
 // narrowed to the arithmetic type inside the kernel.
+// The physics parameters rounded once to FP32 on the host (the FP32 kernels
+// read these instead of converting the doubles in every lane).
+struct PhysF {
+  float tau0, omega0, nu0, les_c, tau_max;
+  float G[3];
+  float K, CF;
+  float inlet_rho, inlet_u[3], outlet_rho;
+};
+
 struct StepArgs {
   const void* fin;
   void* fout;
@@ -25,6 +34,7 @@ struct StepArgs {
   uint32_t n_items;        // number of work items (cells, or chunk cells x chunks)
   uint32_t cshift;         // log2 of the cells per chunk
   uint32_t stream_store;   // select the non-temporal-store kernels (output far above the last-level cache)
+  uint32_t xface;          // compact layout: some run reaches x = 0 or nx - 1 (gather wraps / clamps x there)
   uint32_t N, nx, ny, nz, nxy;
   FastDiv dnx, dny;
   uint32_t per_xcd, nblk;
@@ -34,6 +44,7 @@ struct StepArgs {
   double G[3];
   double K, CF;
   double inlet_rho, inlet_u[3], outlet_rho;
+  PhysF pf;
 };
 
 // Launch one pass.  store_f: write post-collision populations (a step);

@@ -56,6 +56,7 @@ struct Codec;
 template <>
 struct Codec<float> {
   using real = float;
+  __device__ static inline float decode(float v) { return v; }
   __device__ static inline float load(const float* p) { return *p; }
   __device__ static inline void store(float* p, float v) { *p = v; }
   __device__ static inline void store_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
@@ -64,6 +65,7 @@ struct Codec<float> {
 template <>
 struct Codec<double> {
   using real = double;
+  __device__ static inline double decode(double v) { return v; }
   __device__ static inline double load(const double* p) { return *p; }
   __device__ static inline void store(double* p, double v) { *p = v; }
   __device__ static inline void store_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
@@ -72,6 +74,7 @@ struct Codec<double> {
 template <>
 struct Codec<_Float16> {
   using real = float;
+  __device__ static inline float decode(_Float16 v) { return (float)v; }
   __device__ static inline float load(const _Float16* p) { return (float)(*p); }
   __device__ static inline void store(_Float16* p, float v) { *p = (_Float16)v; }
   __device__ static inline void store_nt(_Float16* p, float v) { __builtin_nontemporal_store((_Float16)v, p); }
@@ -86,12 +89,17 @@ __device__ inline uint16_t f32_to_bf16(float v) {
   return (uint16_t)(u >> 16);
 }
 
+// gfx950 converts FP32 -> BF16 in one instruction (v_cvt_pk_bf16_f32,
+// round-to-nearest-even, the rounding of f32_to_bf16 for every finite value)
+__device__ inline uint16_t f32_to_bf16_hw(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
+
 template <>
 struct Codec<bf16_t> {
   using real = float;
+  __device__ static inline float decode(bf16_t v) { return bf16_to_f32(v.bits); }
   __device__ static inline float load(const bf16_t* p) { return bf16_to_f32(p->bits); }
-  __device__ static inline void store(bf16_t* p, float v) { p->bits = f32_to_bf16(v); }
-  __device__ static inline void store_nt(bf16_t* p, float v) { __builtin_nontemporal_store(f32_to_bf16(v), &p->bits); }
+  __device__ static inline void store(bf16_t* p, float v) { p->bits = f32_to_bf16_hw(v); }
+  __device__ static inline void store_nt(bf16_t* p, float v) { __builtin_nontemporal_store(f32_to_bf16_hw(v), &p->bits); }
 };
 
 // ---------------------------------------------------------------------------
@@ -237,6 +245,9 @@ __device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32
   }
   return ok;
 }
+
+// Cell-flag bits 1..18: link q bounces back (bit 0, the rest population, never does).
+constexpr uint32_t BOUNCE_BITS = 0x7fffeu;
 
 // Slot of a compact-layout cell, clamped into the plane: no record content
 // leads a kernel outside the population buffers.

@@ -2,12 +2,117 @@
 // units that instantiate its dense-layout (lbm_step.hip) and compact-layout
 // (lbm_step_sparse.hip) variants.  See lbm_step.hip for the algorithm.
 #pragma once
+#include <type_traits>
 #include "cell_model.h"
 #include "engine.h"
 
 namespace pol {
 
-template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32, bool NT = false, bool SPARSE = false>
+template <typename R>
+__device__ inline Phys<R> make_phys(const StepArgs& a) {
+  Phys<R> P;
+  if constexpr (sizeof(R) == 4) {
+    const PhysF& p = a.pf;  // rounded on the host: no per-lane conversions
+    P.tau0 = p.tau0;
+    P.omega0 = p.omega0;
+    P.nu0 = p.nu0;
+    P.les_c = p.les_c;
+    P.tau_max = p.tau_max;
+    P.G[0] = p.G[0];
+    P.G[1] = p.G[1];
+    P.G[2] = p.G[2];
+    P.K = p.K;
+    P.CF = p.CF;
+    P.inlet_rho = p.inlet_rho;
+    P.inlet_u[0] = p.inlet_u[0];
+    P.inlet_u[1] = p.inlet_u[1];
+    P.inlet_u[2] = p.inlet_u[2];
+    P.outlet_rho = p.outlet_rho;
+  } else {
+    P.tau0 = (R)a.tau0;
+    P.omega0 = R(1) / (R)a.tau0;
+    P.nu0 = (R)a.nu0;
+    P.les_c = (R)a.les_c;
+    P.tau_max = (R)a.tau_max;
+    P.G[0] = (R)a.G[0];
+    P.G[1] = (R)a.G[1];
+    P.G[2] = (R)a.G[2];
+    P.K = (R)a.K;
+    P.CF = (R)a.CF;
+    P.inlet_rho = (R)a.inlet_rho;
+    P.inlet_u[0] = (R)a.inlet_u[0];
+    P.inlet_u[1] = (R)a.inlet_u[1];
+    P.inlet_u[2] = (R)a.inlet_u[2];
+    P.outlet_rho = (R)a.outlet_rho;
+  }
+  return P;
+}
+
+// Compact-layout gather of the 19 incoming populations of one cell.
+//
+// The source of link q is slot rb[row(q)] + x - e_x(q) of plane q.  The row
+// offset rb + x is clamped once per row into [1, P - 2] (the layout's guard
+// slots keep every cell's own source inside that range, so the clamp only
+// bounds the reads of links that bounce) and - e_x is a constant address
+// offset, so every load is one instruction on a scalar plane base with one
+// 32-bit lane offset.  A bounced link reads D*_{opp q} at the lane's own slot
+// instead (w_{opp q} = w_q): the same load, its lane offset switched
+// branch-free (bit q of the flags as a lane mask), so each link stays one
+// load and one memory round trip.  Both links of a pair (q, opp q) address
+// from the lower of their two planes, which keeps every lane offset
+// non-negative.  With XF (some run reaches x = 0 or nx - 1) lanes on an x
+// face wrap (periodic) or clamp x like the dense gather; the layouts of the
+// cup need no such variant, which keeps its branch out of the common kernel.
+template <typename S, bool A32, bool XF>
+__device__ __forceinline__ void gather_compact(typename Codec<S>::real (&f)[Q], const S* __restrict__ fin, uint32_t PS,
+                                               const int (&rb)[9], uint32_t x, uint32_t nx, bool px, uint32_t own,
+                                               uint32_t fl) {
+  using C = Codec<S>;
+  using AI = typename Addr<A32>::idx;  // byte offsets: 32-bit below 4 GiB of populations
+  using SI = std::conditional_t<A32, int32_t, int64_t>;
+  constexpr AI B = sizeof(S);
+  const AI PSB = (AI)PS * B;
+  const int lo1 = 1, hi1 = (int)PS - 2;
+  AI ro[9];  // byte offsets of the nine source rows at this x
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int s = rb[k] + (int)x;
+    ro[k] = (AI)(s < lo1 ? lo1 : (s > hi1 ? hi1 : s)) * B;
+  }
+  int oxm = -1, oxp = 1;
+  bool xface = false;
+  if constexpr (XF) {
+    xface = x == 0 || x == nx - 1;
+    oxm = x == 0 ? (px ? (int)nx - 1 : 0) : -1;             // source at x-1
+    oxp = x == nx - 1 ? (px ? -((int)nx - 1) : 0) : 1;      // source at x+1
+  }
+  const AI own_b = (AI)(own < 1u ? 1u : own) * B;  // >= B: the - e_x offset never wraps
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = OPP(q);
+    constexpr int lo = q < o ? q : o;
+    constexpr int k = (1 - EY(q)) + 3 * (1 - EZ(q));
+    AI off = ro[k];
+    if constexpr (XF && EX(q) != 0) {
+      if (xface) {
+        // wrapped / clamped source, folded back onto the constant - e_x below
+        const int dx = EX(q) > 0 ? oxm : oxp;
+        const int s = rb[k] + (int)x + dx;
+        off = (AI)((s < lo1 ? lo1 : (s > hi1 ? hi1 : s)) + EX(q)) * B;
+      }
+    }
+    if constexpr (q != lo) off += (AI)(q - lo) * PSB;
+    if constexpr (q != 0) {
+      const AI bounce = own_b + (AI)(o - lo) * PSB + (AI)((SI)EX(q) * (SI)B);
+      const AI m = (AI)(SI)__builtin_amdgcn_sbfe((int)fl, q, 1);  // 0 or all ones
+      off = (bounce & m) | (off & ~m);
+    }
+    const S* base = fin + (AI)lo * PS;
+    f[q] = C::decode(*((const S*)((const char*)base + off) - EX(q)));
+  });
+}
+
+template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32, bool NT = false, bool SPARSE = false, bool XF = true>
 __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   static_assert(!SPARSE || (FLAGS && FULL), "the compact layout runs the general kernel");
   using C = Codec<S>;
@@ -33,21 +138,7 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   const uint32_t kind = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;
   const bool active = inside && kind != POL_KIND_SOLID;
 
-  Phys<R> P;
-  P.tau0 = (R)a.tau0;
-  P.nu0 = (R)a.nu0;
-  P.les_c = (R)a.les_c;
-  P.tau_max = (R)a.tau_max;
-  P.G[0] = (R)a.G[0];
-  P.G[1] = (R)a.G[1];
-  P.G[2] = (R)a.G[2];
-  P.K = (R)a.K;
-  P.CF = (R)a.CF;
-  P.inlet_rho = (R)a.inlet_rho;
-  P.inlet_u[0] = (R)a.inlet_u[0];
-  P.inlet_u[1] = (R)a.inlet_u[1];
-  P.inlet_u[2] = (R)a.inlet_u[2];
-  P.outlet_rho = (R)a.outlet_rho;
+  const Phys<R> P = make_phys<R>(a);
 
   const S* __restrict__ fin = (const S*)a.fin;
   S* __restrict__ fout = (S*)a.fout;
@@ -60,7 +151,10 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   R f[Q];  // scaled deviations 36 (f - w)
   CellOut<R> o{R(0), R(0), R(0), R(0)};
   if (active) {
-    if (!FULL || kind != POL_KIND_INLET) {
+    if constexpr (SPARSE) {
+      if (kind != POL_KIND_INLET)
+        gather_compact<S, A32, XF>(f, fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl);
+    } else if (!FULL || kind != POL_KIND_INLET) {
       // neighbour offsets; non-periodic axes clamp to the cell's own plane
       // (zero-gradient for open-boundary cells; walls use bounce-back bits)
       const bool px = a.options & POL_PERIODIC_X, py = a.options & POL_PERIODIC_Y, pz = a.options & POL_PERIODIC_Z;
@@ -75,14 +169,7 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
         constexpr int q = decltype(qc)::value;
         const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
                         (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
-        AI src;
-        if constexpr (SPARSE) {
-          // source row from the chunk record, x shifted like the dense gather
-          const int dx = EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0);
-          src = (AI)q * PS + slot_clamp(rb[(1 - EY(q)) + 3 * (1 - EZ(q))] + (int)x + dx, PS);
-        } else {
-          src = (AI)q * PS + (uint32_t)((int)cell + off);
-        }
+        AI src = (AI)q * PS + (uint32_t)((int)cell + off);
         if constexpr (FLAGS && q != 0) {
           if (fl & (1u << q)) src = (AI)OPP(q) * PS + own;
         }
@@ -113,12 +200,12 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
       if (porous && (a.options & POL_PERM_FIELD)) K = ((const R*)a.perm)[cell];
     }
     const bool les = FULL && (a.options & POL_LES) && !(fl & POL_FLAG_NO_LES);
-    if constexpr (STORE) {
-      o = collide_cell<R, FULL>(f, kind, porous, les, K, fx, fy, fz, bx, by, bz, B, P);
-    } else {
-      // moments only: the same velocity definition the collision uses
-      o = collide_cell<R, FULL>(f, kind, porous, false, K, fx, fy, fz, bx, by, bz, B, P);
-    }
+    // moments-only passes use the same velocity definition the collision uses
+    const bool les_on = STORE && les;
+    if constexpr (sizeof(S) == 2)
+      o = collide_cell_lp<FULL>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P);
+    else
+      o = collide_cell<R, FULL>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P);
   }
   if constexpr (STORE) {
     // Solid lanes next to fluid ones store zero deviations (f = w; no gather
@@ -142,10 +229,11 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
       }
     }
     if (store) {
+      // per-plane scalar base + the lane's one slot offset
       static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (NT) C::store_nt(AD::at(fout, (AI)q * PS + own), f[q]);
-        else C::store(AD::at(fout, (AI)q * PS + own), f[q]);
+        if constexpr (NT) C::store_nt(AD::at(fout + (AI)q * PS, own), f[q]);
+        else C::store(AD::at(fout + (AI)q * PS, own), f[q]);
       });
     }
   }

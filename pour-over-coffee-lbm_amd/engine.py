@@ -170,6 +170,7 @@ class HipContext:
             b.chunk_records = self.records.data_ptr()
             b.n_chunks = self.layout.n_chunks
             b.stored_cells = self.layout.stored_cells
+            b.layout_flags = 0 if self.layout.x_face else L.POL_LAYOUT_NO_X_FACE
             b.chunk_cells = self.chunk_cells
         elif self.chunks is not None:
             b.active_chunks = self.chunks.data_ptr()

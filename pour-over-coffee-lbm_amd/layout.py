@@ -9,11 +9,15 @@ order:
 
     population q of cell (x, y, z) = plane q, slot row_base[z*ny + y] + x
 
-with plane stride M = stored_cells (the run lengths summed, rounded up to 64
-slots so every plane starts on a 256-byte boundary).  Every stored slot is
-written each step (non-solid cells with their populations, solid cells inside
-a run with zero deviations), so written lines are whole without any solid
-slot outside the runs.
+with plane stride M = stored_cells (a leading guard of 64 slots,
+the run lengths summed, at least two trailing guard slots, rounded up to 64
+slots so every plane starts on a 256-byte boundary).  Every stored slot of a
+run is written each step (non-solid cells with their populations, solid
+cells inside a run with zero deviations), so written lines are whole without
+any solid slot outside the runs.  The guards keep slots 0, 1, M - 2 and
+M - 1 free of cells: a kernel may then clamp a row offset (source slot + e_x)
+into [1, M - 2] once and reach the x - 1 / x + 1 neighbours with constant
+offsets, without the clamp ever moving a cell's own source.
 
 A launch uses chunk records instead of the dense chunk list: one record of
 RECORD_WORDS uint32 per 32-cell row chunk that meets a run,
@@ -49,12 +53,13 @@ class CompactLayout:
     ny: int
     nz: int
     stored_cells: int        # M: plane stride in slots
-    used_cells: int          # slots of the stored runs (padding included)
+    used_cells: int          # leading guard + slots of the stored runs (padding included)
     row_base: np.ndarray     # int64 [nz*ny]
     row_run: np.ndarray      # uint32 [nz*ny]: x_begin | x_end << 16 (0 = empty row); x_end may pass nx (padding)
     records: np.ndarray      # uint32 [n_chunks, RECORD_WORDS]
-    slot_dense: np.ndarray   # int64 [used_cells]: dense cell of slot s, -1 for padding
+    slot_dense: np.ndarray   # int64 [used_cells]: dense cell of slot s, -1 for guard and padding
     align: int = 1           # slots a row's storage is aligned and padded to
+    x_face: bool = True      # some run holds a cell at x = 0 or x = nx - 1
 
     @property
     def n_chunks(self) -> int:
@@ -114,20 +119,24 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
     xe = np.where(has, nx - open_cell[:, ::-1].argmax(axis=1), 0).astype(np.int64)
     length = xe - xb
     padded = -(-length // align) * align
-    start = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
-    used = int(padded.sum())
-    if used == 0:
+    # a whole PLANE_ALIGN block: the guard shifts no row off its 128-byte
+    # line parity (a shift by half a line costs each wave a third line)
+    guard = -(-max(align, 2) // PLANE_ALIGN) * PLANE_ALIGN
+    start = guard + np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
+    if int(padded.sum()) == 0:
         raise ValueError("no non-solid cell: nothing to store")
-    M = -(-used // PLANE_ALIGN) * PLANE_ALIGN
+    used = guard + int(padded.sum())
+    M = -(-(used + 2) // PLANE_ALIGN) * PLANE_ALIGN
     if M >= 1 << 31:
         raise ValueError("too many stored cells for 32-bit slots")
     row_base = np.where(has, start - xb, 0).astype(np.int64)
     row_run = (xb | ((xb + padded) << 16)).astype(np.uint32)
     row_run[~has] = 0
-    # dense cell of every slot (-1 for padding)
+    # dense cell of every slot (-1 for guard and padding slots)
     rows = np.repeat(np.arange(nz * ny, dtype=np.int64), padded)
-    offs = np.arange(used, dtype=np.int64) - np.repeat(start, padded)
-    slot_dense = np.where(offs < np.repeat(length, padded), rows * nx + np.repeat(xb, padded) + offs, -1)
+    offs = np.arange(guard, used, dtype=np.int64) - np.repeat(start, padded)
+    slot_dense = np.concatenate([np.full(guard, -1, np.int64),
+                                 np.where(offs < np.repeat(length, padded), rows * nx + np.repeat(xb, padded) + offs, -1)])
     if align == 1:
         # chunk_cells-cell row chunks (x0 a multiple of chunk_cells) that meet a
         # row's run (so every stored slot is written each step)
@@ -155,4 +164,5 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
             rec[:, 1 + a + 3 * b] = row_base[z2 * ny + y2]
     rec[:, 10] = row_run[rr]
     records = (rec & 0xFFFFFFFF).astype(np.uint32)  # row bases as two's-complement int32
-    return CompactLayout(nx, ny, nz, M, used, row_base, row_run, records, slot_dense, align)
+    x_face = bool((has & ((xb == 0) | (xe == nx))).any())
+    return CompactLayout(nx, ny, nz, M, used, row_base, row_run, records, slot_dense, align, x_face)

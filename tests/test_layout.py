@@ -97,6 +97,13 @@ def test_gather_slots_match_dense_sources(name, align, chunk):
         m = sel & ~bounce
         assert (kind[zs[m], ys[m], xs[m]] != L.POL_KIND_SOLID).all(), q
         assert np.array_equal(slot[m], lay.slot_of(xs[m], ys[m], zs[m])), q
+        # the kernel's fast form for lanes off the x faces: the row offset
+        # clamped once into [1, M - 2], the x shift a constant; every link
+        # (bounce links too, whose value is replaced) stays inside the plane
+        fast = sel & (x > 0) & (x < nx - 1)
+        ro = np.clip(_rb(lay, r, 1 - ey, 1 - ez) + x, 1, lay.stored_cells - 2)
+        assert np.array_equal((ro - ex)[fast & ~bounce], slot[fast & ~bounce]), q
+        assert (ro - ex).min() >= 0 and (ro - ex).max() <= lay.stored_cells - 1
 
 
 @pytest.mark.parametrize("align,chunk", LAYOUTS)
@@ -138,7 +145,10 @@ def test_dense_round_trip_and_every_slot_written(align, chunk):
     run = lay.records[r, 10].astype(np.int64)
     inr = (sx >= (run & 0xFFFF)) & (sx < (run >> 16))
     slots = np.sort(lay.row_base[z[inr] * lay.ny + y[inr]] + sx[inr])
-    assert np.array_equal(slots, np.arange(lay.used_cells))
+    guard = max(64, align)
+    assert np.array_equal(slots, np.arange(guard, lay.used_cells))
+    # guard slots at both plane ends hold no cell
+    assert (lay.slot_dense[:guard] < 0).all() and lay.used_cells <= lay.stored_cells - 2
     # a padding lane is never a valid cell of the grid
     pad = inr & (lay.slot_dense[np.clip(lay.row_base[z * lay.ny + y] + sx, 0, lay.used_cells - 1)] < 0)
     assert not (pad & ok & (sx < lay.nx) & (g.kind.reshape(-1)[np.clip((z * lay.ny + y) * lay.nx + sx, 0, g.kind.size - 1)]
@@ -150,4 +160,15 @@ def test_dense_round_trip_and_every_slot_written(align, chunk):
         assert lay.used_cells % align == 0
     if align == 1:
         # packed runs: one slot per run cell
-        assert lay.used_cells == int(((lay.row_run >> 16) - (lay.row_run & 0xFFFF)).astype(np.int64).sum())
+        assert lay.used_cells == 64 + int(((lay.row_run >> 16) - (lay.row_run & 0xFFFF)).astype(np.int64).sum())
+
+
+def test_x_face_flag():
+    """x_face is set exactly when some run holds a cell on an x face (the
+    kernels need the x wrap / clamp variant of the gather then)."""
+    g = G.build_v60(G.V60Params.scaled(48))
+    lay = build_compact_layout(g.flags, chunk_cells=16, align=16)
+    open_cell = g.kind != L.POL_KIND_SOLID
+    assert lay.x_face == bool(open_cell[:, :, 0].any() or open_cell[:, :, -1].any())
+    kind, flags, rev, per = _geometries()["periodic_xy"]
+    assert build_compact_layout(flags, per, chunk_cells=16, align=16).x_face
