@@ -20,6 +20,7 @@
 // There is no adjoint in the reference; the forward semantics it inverts are
 // those of lbm_step.hip / cell_model.h.
 #include "cell_model.h"
+#include "compact_gather.h"
 #include "engine.h"
 
 namespace pol {
@@ -75,7 +76,7 @@ struct AdjGeom {
   bool px, py, pz;
 };
 
-template <typename S, typename R, bool A32, bool SPARSE>
+template <typename S, typename R, bool A32, bool SPARSE, bool XF = true>
 __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const uint32_t* __restrict__ rflags) {
   using Cd = Codec<S>;
   using AD = Addr<A32>;
@@ -127,7 +128,15 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
     const R* __restrict__ mu_in = (const R*)A.lam_in;
     // ---- 1. lambda_p = dJ/df^{n+1}_p(x), reverse gather of mu^{n+1} ----
     const uint32_t rf = rflags[cell];
-    static_for<0, Q>([&](auto qc) {
+    if constexpr (SPARSE) {
+      // one load per direction from the selected reader (compact_gather.h)
+      compact_readers<R, A32, XF>(mu_in, P, rb, x, a.nx, px, own, rf, [&](auto pc, R v) {
+        constexpr int p = decltype(pc)::value;
+        bool bb = false;
+        if constexpr (p != 0) bb = kind != POL_KIND_INLET && (fl & (1u << OPP(p)));
+        lam[p] = ((rf & (1u << p)) || bb) ? v : R(0);
+      });
+    } else static_for<0, Q>([&](auto qc) {
       constexpr int p = decltype(qc)::value;
       // normal reader at x + e_p
       const int off = (EX(p) > 0 ? oxp : (EX(p) < 0 ? oxm : 0)) + (EY(p) > 0 ? oyp : (EY(p) < 0 ? oym : 0)) +
@@ -140,13 +149,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       const bool nb = rf & (1u << p);
       bool bb = false;
       if constexpr (p != 0) bb = kind != POL_KIND_INLET && (fl & (1u << OPP(p)));
-      uint32_t rslot;
-      if constexpr (SPARSE) {
-        const int dx = EX(p) > 0 ? oxp : (EX(p) < 0 ? oxm : 0);
-        rslot = slot_clamp(rb[(1 + EY(p)) + 3 * (1 + EZ(p))] + (int)x + dx, P);
-      } else {
-        rslot = (uint32_t)((int)cell + off);
-      }
+      const uint32_t rslot = (uint32_t)((int)cell + off);
       const AI src = nb ? (AI)p * P + rslot : (AI)OPP(p) * P + own;
       const R v = *AD::at(mu_in, src);
       lam[p] = (nb || bb) ? v : R(0);
@@ -212,21 +215,8 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       // live beside lam.
       R s36 = 0, mx = 0, my = 0, mz = 0, lamD = 0;
       R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
-      static_for<0, Q>([&](auto qc) {
+      auto reduce = [&](auto qc, R d) {
         constexpr int q = decltype(qc)::value;
-        const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
-                        (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
-        AI src;
-        if constexpr (SPARSE) {
-          const int dx = EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0);
-          src = (AI)q * P + slot_clamp(rb[(1 - EY(q)) + 3 * (1 - EZ(q))] + (int)x + dx, P);
-        } else {
-          src = (AI)q * P + (uint32_t)((int)cell + off);
-        }
-        if constexpr (q != 0) {
-          if (fl & (1u << q)) src = (AI)OPP(q) * P + own;
-        }
-        const R d = Cd::load(AD::at(fin, src));
         s36 += d;
         lamD += lam[q] * d;
         if constexpr (EX(q) > 0) mx += d;
@@ -244,7 +234,21 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         if constexpr (EX(q) * EZ(q) < 0) pxz -= d;
         if constexpr (EY(q) * EZ(q) > 0) pyz += d;
         if constexpr (EY(q) * EZ(q) < 0) pyz -= d;
-      });
+      };
+      if constexpr (SPARSE) {
+        compact_sources<S, A32, XF>(fin, P, rb, x, a.nx, px, own, fl, reduce);
+      } else {
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
+                          (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
+          AI src = (AI)q * P + (uint32_t)((int)cell + off);
+          if constexpr (q != 0) {
+            if (fl & (1u << q)) src = (AI)OPP(q) * P + own;
+          }
+          reduce(qc, Cd::load(AD::at(fin, src)));
+        });
+      }
       // the linearisation point of the adjoint: a rounded 1/36 is immaterial
       const R i36 = R(1.0 / 36.0);
       const R rd = s36 * i36, rho = R(1) + rd;
@@ -440,7 +444,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         eacc<EX(q) * EY(q)>(ePe, Pxy2);
         eacc<EX(q) * EZ(q)>(ePe, Pxz2);
         eacc<EY(q) * EZ(q)>(ePe, Pyz2);
-        *AD::at(mu_out, (AI)q * P + own) = keep * lam[q] + C + edot<EX(q), EY(q), EZ(q)>(hx, hy, hz) + L1 * ePe;
+        *AD::at(mu_out + (AI)q * P, own) = keep * lam[q] + C + edot<EX(q), EY(q), EZ(q)>(hx, hy, hz) + L1 * ePe;
       });
     }
   }
@@ -477,10 +481,21 @@ __global__ void __launch_bounds__(256) k_adjoint_final(const double* __restrict_
 
 template <typename S, bool SPARSE>
 static void launch_adj(const AdjointArgs& A, const uint32_t* rflags, dim3 grid, hipStream_t s) {
-  if (fits_addr32(Q, A.fw.P, sizeof(S)))
+  const bool a32 = fits_addr32(Q, A.fw.P, sizeof(S));
+  if constexpr (SPARSE) {
+    if (!A.fw.xface) {  // compact layout with no run on an x face
+      if (a32)
+        hipLaunchKernelGGL((k_adjoint<S, S, true, true, false>), grid, dim3(BLOCK), 0, s, A, rflags);
+      else
+        hipLaunchKernelGGL((k_adjoint<S, S, false, true, false>), grid, dim3(BLOCK), 0, s, A, rflags);
+      return;
+    }
+  }
+  if (a32) {
     hipLaunchKernelGGL((k_adjoint<S, S, true, SPARSE>), grid, dim3(BLOCK), 0, s, A, rflags);
-  else
+  } else {
     hipLaunchKernelGGL((k_adjoint<S, S, false, SPARSE>), grid, dim3(BLOCK), 0, s, A, rflags);
+  }
 }
 
 hipError_t launch_adjoint(int storage, const AdjointArgs& A, const uint32_t* rflags, const Grid& g, hipStream_t s) {

@@ -1,0 +1,132 @@
+// Gathers of the compact row-run layout, shared by the forward stream-collide
+// kernel and the adjoint kernel.
+//
+// Both walk the 19 links of one cell and hand each loaded value to a
+// callback use(q, value) (q as std::integral_constant), so the forward kernel
+// keeps the values and the adjoint reduces them as they arrive.
+//
+// Addressing.  A link's source (forward: x - e_q; reverse: x + e_p) lies in
+// one of the nine rows (y +- 1, z +- 1) of the cell's chunk record, at column
+// x -+ 1.  The row offset rb + x is clamped once per row into [1, P - 2]: the
+// layout's guard slots (slots 0, 1, P - 2 and P - 1 hold no cell) keep every
+// legitimate source inside that range, so the clamp only bounds the reads of
+// links whose value is replaced.  The column shift -+ 1 is a constant address
+// offset, so a link is one load on a scalar plane base with one 32-bit lane
+// offset.  A link that reads the cell itself instead (forward: a bounced link
+// q reads D*_{opp q}(x); reverse: the reader of f_p(x) is x's own bounce link
+// opp p) switches the lane offset branch-free, with a flag bit as lane mask,
+// to the own slot of the other plane: still one load, one memory round trip.
+// Both links of a pair (q, opp q) address from the lower of the two planes,
+// so every lane offset is non-negative.  XF (some run reaches x = 0 or
+// nx - 1) adds the x-face wrap (periodic) or clamp of the dense gather.
+#pragma once
+#include <type_traits>
+
+#include "cell_model.h"
+
+namespace pol {
+
+template <bool A32>
+struct CompactRows {
+  using AI = typename Addr<A32>::idx;  // byte offsets: 32-bit below 4 GiB of populations
+  AI ro[9];      // byte offset of row k at this x (clamped slot rb[k] + x)
+  AI own_b;      // byte offset of the own slot, at least one element
+  int lo1, hi1;  // clamp range [1, P - 2]
+
+  template <typename T>
+  __device__ CompactRows(const int (&rb)[9], uint32_t x, uint32_t own, uint32_t PS, T*) {
+    constexpr AI B = sizeof(T);
+    lo1 = 1;
+    hi1 = (int)PS - 2;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) ro[k] = (AI)clampi(rb[k] + (int)x) * B;
+    own_b = (AI)(own < 1u ? 1u : own) * B;
+  }
+  __device__ int clampi(int s) const { return s < lo1 ? lo1 : (s > hi1 ? hi1 : s); }
+};
+
+// Forward (pull) sources of a cell: link q reads slot rb[(1 - e_y) + 3 (1 - e_z)]
+// + x - e_x of plane q, or D*_{opp q} at its own slot when bit q of the cell
+// flags is set.
+template <typename S, bool A32, bool XF, typename Use>
+__device__ __forceinline__ void compact_sources(const S* __restrict__ fin, uint32_t PS, const int (&rb)[9], uint32_t x,
+                                                uint32_t nx, bool px, uint32_t own, uint32_t fl, Use&& use) {
+  using C = Codec<S>;
+  using AI = typename Addr<A32>::idx;
+  using SI = std::conditional_t<A32, int32_t, int64_t>;
+  constexpr AI B = sizeof(S);
+  const AI PSB = (AI)PS * B;
+  const CompactRows<A32> R(rb, x, own, PS, fin);
+  bool xface = false;
+  int oxm = -1, oxp = 1;
+  if constexpr (XF) {
+    xface = x == 0 || x == nx - 1;
+    oxm = x == 0 ? (px ? (int)nx - 1 : 0) : -1;         // source at x-1
+    oxp = x == nx - 1 ? (px ? -((int)nx - 1) : 0) : 1;  // source at x+1
+  }
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = OPP(q);
+    constexpr int lo = q < o ? q : o;
+    constexpr int k = (1 - EY(q)) + 3 * (1 - EZ(q));
+    AI off = R.ro[k];
+    if constexpr (XF && EX(q) != 0) {
+      if (xface) {  // wrapped / clamped source, folded back onto the constant - e_x
+        const int dx = EX(q) > 0 ? oxm : oxp;
+        off = (AI)(R.clampi(rb[k] + (int)x + dx) + EX(q)) * B;
+      }
+    }
+    if constexpr (q != lo) off += (AI)(q - lo) * PSB;
+    if constexpr (q != 0) {
+      const AI bounce = R.own_b + (AI)(o - lo) * PSB + (AI)((SI)EX(q) * (SI)B);
+      const AI m = (AI)(SI)__builtin_amdgcn_sbfe((int)fl, q, 1);  // 0 or all ones
+      off = (bounce & m) | (off & ~m);
+    }
+    const S* base = fin + (AI)lo * PS;
+    use(qc, C::decode(*((const S*)((const char*)base + off) - EX(q))));
+  });
+}
+
+// Reverse (adjoint) readers of a cell's populations: f_p(x) is read either by
+// x + e_p (normal link; bit p of the reverse flags rf) at slot
+// rb[(1 + e_y) + 3 (1 + e_z)] + x + e_x of plane p, or by x itself through its
+// bounce link opp p (slot own of plane opp p).  The value handed to use() is
+// the one of the selected address; the caller zeroes it when neither reader
+// exists.
+template <typename T, bool A32, bool XF, typename Use>
+__device__ __forceinline__ void compact_readers(const T* __restrict__ mu, uint32_t PS, const int (&rb)[9], uint32_t x,
+                                                uint32_t nx, bool px, uint32_t own, uint32_t rf, Use&& use) {
+  using AI = typename Addr<A32>::idx;
+  using SI = std::conditional_t<A32, int32_t, int64_t>;
+  constexpr AI B = sizeof(T);
+  const AI PSB = (AI)PS * B;
+  const CompactRows<A32> R(rb, x, own, PS, mu);
+  bool xface = false;
+  int oxm = -1, oxp = 1;
+  if constexpr (XF) {
+    xface = x == 0 || x == nx - 1;
+    oxm = x == 0 ? (px ? (int)nx - 1 : 0) : -1;         // reader at x-1
+    oxp = x == nx - 1 ? (px ? -((int)nx - 1) : 0) : 1;  // reader at x+1
+  }
+  static_for<0, Q>([&](auto pc) {
+    constexpr int p = decltype(pc)::value;
+    constexpr int o = OPP(p);
+    constexpr int lo = p < o ? p : o;
+    constexpr int k = (1 + EY(p)) + 3 * (1 + EZ(p));
+    AI off = R.ro[k];
+    if constexpr (XF && EX(p) != 0) {
+      if (xface) {
+        const int dx = EX(p) > 0 ? oxp : oxm;
+        off = (AI)(R.clampi(rb[k] + (int)x + dx) - EX(p)) * B;
+      }
+    }
+    if constexpr (p != lo) off += (AI)(p - lo) * PSB;
+    const AI bounce = R.own_b + (AI)(o - lo) * PSB - (AI)((SI)EX(p) * (SI)B);
+    const AI m = (AI)(SI)__builtin_amdgcn_sbfe((int)rf, p, 1);  // normal reader: 0 -> own slot
+    off = (off & m) | (bounce & ~m);
+    const T* base = mu + (AI)lo * PS;
+    use(pc, *((const T*)((const char*)base + off) + EX(p)));
+  });
+}
+
+}  // namespace pol

@@ -2,8 +2,8 @@
 // units that instantiate its dense-layout (lbm_step.hip) and compact-layout
 // (lbm_step_sparse.hip) variants.  See lbm_step.hip for the algorithm.
 #pragma once
-#include <type_traits>
 #include "cell_model.h"
+#include "compact_gather.h"
 #include "engine.h"
 
 namespace pol {
@@ -48,70 +48,6 @@ __device__ inline Phys<R> make_phys(const StepArgs& a) {
   return P;
 }
 
-// Compact-layout gather of the 19 incoming populations of one cell.
-//
-// The source of link q is slot rb[row(q)] + x - e_x(q) of plane q.  The row
-// offset rb + x is clamped once per row into [1, P - 2] (the layout's guard
-// slots keep every cell's own source inside that range, so the clamp only
-// bounds the reads of links that bounce) and - e_x is a constant address
-// offset, so every load is one instruction on a scalar plane base with one
-// 32-bit lane offset.  A bounced link reads D*_{opp q} at the lane's own slot
-// instead (w_{opp q} = w_q): the same load, its lane offset switched
-// branch-free (bit q of the flags as a lane mask), so each link stays one
-// load and one memory round trip.  Both links of a pair (q, opp q) address
-// from the lower of their two planes, which keeps every lane offset
-// non-negative.  With XF (some run reaches x = 0 or nx - 1) lanes on an x
-// face wrap (periodic) or clamp x like the dense gather; the layouts of the
-// cup need no such variant, which keeps its branch out of the common kernel.
-template <typename S, bool A32, bool XF>
-__device__ __forceinline__ void gather_compact(typename Codec<S>::real (&f)[Q], const S* __restrict__ fin, uint32_t PS,
-                                               const int (&rb)[9], uint32_t x, uint32_t nx, bool px, uint32_t own,
-                                               uint32_t fl) {
-  using C = Codec<S>;
-  using AI = typename Addr<A32>::idx;  // byte offsets: 32-bit below 4 GiB of populations
-  using SI = std::conditional_t<A32, int32_t, int64_t>;
-  constexpr AI B = sizeof(S);
-  const AI PSB = (AI)PS * B;
-  const int lo1 = 1, hi1 = (int)PS - 2;
-  AI ro[9];  // byte offsets of the nine source rows at this x
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    const int s = rb[k] + (int)x;
-    ro[k] = (AI)(s < lo1 ? lo1 : (s > hi1 ? hi1 : s)) * B;
-  }
-  int oxm = -1, oxp = 1;
-  bool xface = false;
-  if constexpr (XF) {
-    xface = x == 0 || x == nx - 1;
-    oxm = x == 0 ? (px ? (int)nx - 1 : 0) : -1;             // source at x-1
-    oxp = x == nx - 1 ? (px ? -((int)nx - 1) : 0) : 1;      // source at x+1
-  }
-  const AI own_b = (AI)(own < 1u ? 1u : own) * B;  // >= B: the - e_x offset never wraps
-  static_for<0, Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int o = OPP(q);
-    constexpr int lo = q < o ? q : o;
-    constexpr int k = (1 - EY(q)) + 3 * (1 - EZ(q));
-    AI off = ro[k];
-    if constexpr (XF && EX(q) != 0) {
-      if (xface) {
-        // wrapped / clamped source, folded back onto the constant - e_x below
-        const int dx = EX(q) > 0 ? oxm : oxp;
-        const int s = rb[k] + (int)x + dx;
-        off = (AI)((s < lo1 ? lo1 : (s > hi1 ? hi1 : s)) + EX(q)) * B;
-      }
-    }
-    if constexpr (q != lo) off += (AI)(q - lo) * PSB;
-    if constexpr (q != 0) {
-      const AI bounce = own_b + (AI)(o - lo) * PSB + (AI)((SI)EX(q) * (SI)B);
-      const AI m = (AI)(SI)__builtin_amdgcn_sbfe((int)fl, q, 1);  // 0 or all ones
-      off = (bounce & m) | (off & ~m);
-    }
-    const S* base = fin + (AI)lo * PS;
-    f[q] = C::decode(*((const S*)((const char*)base + off) - EX(q)));
-  });
-}
-
 template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32, bool NT = false, bool SPARSE = false, bool XF = true>
 __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   static_assert(!SPARSE || (FLAGS && FULL), "the compact layout runs the general kernel");
@@ -153,7 +89,8 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   if (active) {
     if constexpr (SPARSE) {
       if (kind != POL_KIND_INLET)
-        gather_compact<S, A32, XF>(f, fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl);
+        compact_sources<S, A32, XF>(fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl,
+                                    [&](auto qc, R v) { f[decltype(qc)::value] = v; });
     } else if (!FULL || kind != POL_KIND_INLET) {
       // neighbour offsets; non-periodic axes clamp to the cell's own plane
       // (zero-gradient for open-boundary cells; walls use bounce-back bits)
