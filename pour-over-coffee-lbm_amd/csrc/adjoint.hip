@@ -101,6 +101,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   const uint32_t z = fdiv(yz, a.dny);
   const uint32_t y = yz - z * a.ny;
   const uint32_t fl = inside ? a.flags[cell] : (POL_KIND_SOLID << POL_KIND_SHIFT);
+  const uint32_t rf = inside ? rflags[cell] : 0u;  // loaded beside fl: one round trip for both
   const uint32_t kind = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;
   const bool active = inside && kind != POL_KIND_SOLID;
 
@@ -127,7 +128,6 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   if (active) {
     const R* __restrict__ mu_in = (const R*)A.lam_in;
     // ---- 1. lambda_p = dJ/df^{n+1}_p(x), reverse gather of mu^{n+1} ----
-    const uint32_t rf = rflags[cell];
     if constexpr (SPARSE) {
       // one load per direction from the selected reader (compact_gather.h)
       compact_readers<R, A32, XF>(mu_in, P, rb, x, a.nx, px, own, rf, [&](auto pc, R v) {
@@ -154,6 +154,29 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       const R v = *AD::at(mu_in, src);
       lam[p] = (nb || bb) ? v : R(0);
     });
+    // ---- 1b. re-gather the forward cell from the stored state ----
+    // Issued right behind the reverse gather for every active lane (inlet
+    // lanes ignore the values), so the two gathers share one memory round
+    // trip instead of following each other.
+    R dv[Q];  // stored deviations D_q = 36 (g_q - w_q) of the gathered cell
+    {
+      const S* __restrict__ fin = (const S*)a.fin;
+      if constexpr (SPARSE) {
+        compact_sources<S, A32, XF>(fin, P, rb, x, a.nx, px, own, fl,
+                                    [&](auto qc, R v) { dv[decltype(qc)::value] = v; });
+      } else {
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
+                          (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
+          AI src = (AI)q * P + (uint32_t)((int)cell + off);
+          if constexpr (q != 0) {
+            if (fl & (1u << q)) src = (AI)OPP(q) * P + own;
+          }
+          dv[q] = Cd::load(AD::at(fin, src));
+        });
+      }
+    }
     if (rf & POL_RFLAG_CLAMP) {
       // readers whose out-of-domain source was clamped onto this cell
       const int c[3] = {(int)x, (int)y, (int)z};
@@ -208,15 +231,13 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       part[2] = gz;
       // nothing gathered: mu at inlet cells is never read (left zero)
     } else {
-      const S* __restrict__ fin = (const S*)a.fin;
-      // Stream the gathered cell's stored deviations D_q = 36 (g_q - w_q)
-      // once, reducing them on the fly to what the adjoint needs: moments,
-      // the second moment (LES stress) and sum_q lam_q D_q.  No D array stays
-      // live beside lam.
+      // reduce the gathered deviations to what the adjoint needs: moments,
+      // the second moment (LES stress) and sum_q lam_q D_q
       R s36 = 0, mx = 0, my = 0, mz = 0, lamD = 0;
       R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
-      auto reduce = [&](auto qc, R d) {
+      static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
+        const R d = dv[q];
         s36 += d;
         lamD += lam[q] * d;
         if constexpr (EX(q) > 0) mx += d;
@@ -234,21 +255,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         if constexpr (EX(q) * EZ(q) < 0) pxz -= d;
         if constexpr (EY(q) * EZ(q) > 0) pyz += d;
         if constexpr (EY(q) * EZ(q) < 0) pyz -= d;
-      };
-      if constexpr (SPARSE) {
-        compact_sources<S, A32, XF>(fin, P, rb, x, a.nx, px, own, fl, reduce);
-      } else {
-        static_for<0, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          const int off = (EX(q) > 0 ? oxm : (EX(q) < 0 ? oxp : 0)) + (EY(q) > 0 ? oym : (EY(q) < 0 ? oyp : 0)) +
-                          (EZ(q) > 0 ? ozm : (EZ(q) < 0 ? ozp : 0));
-          AI src = (AI)q * P + (uint32_t)((int)cell + off);
-          if constexpr (q != 0) {
-            if (fl & (1u << q)) src = (AI)OPP(q) * P + own;
-          }
-          reduce(qc, Cd::load(AD::at(fin, src)));
-        });
-      }
+      });
       // the linearisation point of the adjoint: a rounded 1/36 is immaterial
       const R i36 = R(1.0 / 36.0);
       const R rd = s36 * i36, rho = R(1) + rd;
