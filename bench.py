@@ -4,16 +4,18 @@ Workload (one step = one fused stream-collide launch): the 224^3 V60 brew
 domain in FP32 -- 30 deg cone, outlet hole radius 8, 2-cell porous filter
 (Darcy K 1e-3, Forchheimer 0.55), Guo gravity 1e-5, Smagorinsky 0.17,
 inlet jet radius 4 at u_z = -0.03.  Populations start at rest and are
-resident in HBM before timing.  `value` = fluid-cell updates per second
-(MLUPS); the 67 % of cells that are solid get no threads except the solid
-tails of launched 32-cell row chunks.
+resident in HBM before timing, in the compact row-run layout (each row
+stores only the run between its first and last non-solid cell).  `value` =
+fluid-cell updates per second (MLUPS); the 67 % of cells that are solid get
+no threads except the solid cells inside a row's run.
 
 Secondary lines in the same JSON object: the all-fluid periodic 224^3 box
 (every cell updated), FP16/BF16 storage speed-up over FP32, the two-way
 coupled step with 50 000 particles, adjoint cost per forward step, and the
 CPU baseline (C restatements on the host cores, bounded samples).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--precision P]
+       [--coupled-only | --adjoint-only] [--calibrate-copy]
 """
 from __future__ import annotations
 
@@ -219,9 +221,11 @@ def cpu_baseline(budget_s=10.0, single_budget_s=6.0):
 
 def bench_adjoint(n=128, steps=100, reps=20):
     """Adjoint cost at the differentiation workload's grid (scaled V60, FP32):
-    kernel time of one reverse step (forward recompute of the state + adjoint
-    kernel) over one forward step, both from HIP events on the launch stream,
-    and the whole checkpointed gradient over a plain forward run (wall)."""
+    kernel time of one reverse step over one forward step, both from HIP
+    events on the launch stream -- with every forward state kept (the
+    default store at this size) the adjoint kernel alone, with checkpoints
+    every 50 steps one forward recompute more -- and the whole gradient over
+    a plain forward run (wall), for both stores."""
     import torch
 
     from pourover_amd import _lib as L
@@ -267,13 +271,14 @@ def bench_adjoint(n=128, steps=100, reps=20):
     torch.cuda.empty_cache()
     out = {"grid": [n, n, n], "precision": "f32", "forward_kernel_ms": round(fwd_ms, 4),
            "adjoint_kernel_ms": round(adj_ms, 4),
-           "adjoint_step_over_forward": round((fwd_ms + adj_ms) / fwd_ms, 3),
-           "adjoint_step_over_forward_states_kept": round(adj_ms / fwd_ms, 3),
-           "gradient_over_forward_wall": round(t_grad / t_fwd, 2),
-           "gradient_over_forward_wall_states_kept": round(t_grad_all / t_fwd, 2),
+           "default_state_store": "every state kept (checkpoint='auto': the states fit in HBM)",
+           "adjoint_step_over_forward": round(adj_ms / fwd_ms, 3),
+           "adjoint_step_over_forward_checkpointed": round((fwd_ms + adj_ms) / fwd_ms, 3),
+           "gradient_over_forward_wall": round(t_grad_all / t_fwd, 2),
+           "gradient_over_forward_wall_checkpointed": round(t_grad / t_fwd, 2),
            "steps": steps, "checkpoint_every": 50,
-           "note": "adjoint step = forward recompute of the state from its checkpoint + adjoint kernel; "
-                   "with every state kept the step is the adjoint kernel alone"}
+           "note": "with every forward state kept a reverse step is the adjoint kernel alone; with "
+                   "checkpoints every 50 steps it also recomputes one forward step"}
     return out
 
 

@@ -6,13 +6,19 @@ collides; the reduction slot POL_RED_OUTFLUX).  Parameters: the per-cell
 permeability K of the filter layer and the inlet velocity u_in.
 
 The reverse sweep runs csrc/adjoint.hip one step at a time, newest first.
-Forward states are rebuilt from checkpoints every `checkpoint` steps: a
-segment is re-run forward from its checkpoint into a segment store (one
-fused step per state, pol_step_from) and then swept in reverse, so memory
-holds the checkpoints plus one segment (128^3, 50-step segments: 8 GB in
-FP32), far inside 288 GB of HBM.  With checkpoint=None every forward state is
-kept (128^3 x 500 steps: 80 GB in FP32), the forward run writes them as it
-goes and the reverse sweep recomputes nothing.
+Forward states come from one of two stores:
+  - checkpoint=None: every forward state is kept (the 128^3 cup in the
+    compact layout: 56 MB per FP32 state, 28 GB for 500 steps), the forward
+    run writes them as it goes and the reverse sweep recomputes nothing: a
+    reverse step costs one adjoint kernel;
+  - checkpoint=K: a state every K steps; a segment is re-run forward from
+    its checkpoint into a segment store (one fused step per state,
+    pol_step_from) and then swept in reverse, so memory holds the checkpoints
+    plus one segment, and a reverse step costs one forward step more.
+The default, checkpoint="auto", keeps every state when they fit in half of
+the free device memory (on 288 GB of HBM that covers the 128^3 cup for
+thousands of steps) and checkpoints every 50 steps (or the largest divisor
+of `steps` below that) otherwise.
 
 There is no adjoint in the reference (no autograd or checkpointing exists in
 src/); OutletFluxFunction wraps the sweep as a torch.autograd.Function so
@@ -22,7 +28,7 @@ from __future__ import annotations
 
 import ctypes as C
 import time
-from typing import Optional, Sequence
+from typing import Optional, Sequence, Union
 
 import numpy as np
 import torch
@@ -32,22 +38,34 @@ from .engine import HipContext
 from .workloads import V60Workload
 
 
+def auto_checkpoint(steps: int, state_bytes: int, free_bytes: int) -> Optional[int]:
+    """None (keep every state) when steps states fit in half of free_bytes,
+    else the largest divisor of steps not above 50."""
+    if steps * state_bytes <= free_bytes // 2:
+        return None
+    return max(d for d in range(1, min(50, steps) + 1) if steps % d == 0)
+
+
 class AdjointProblem:
     def __init__(self, wl: V60Workload, K: np.ndarray, steps: int = 500, window: int = 100,
-                 checkpoint: Optional[int] = 50, compact: Optional[bool] = None):
+                 checkpoint: Union[int, None, str] = "auto", compact: Optional[bool] = None):
         cfg = wl.config
         if not cfg.perm_field:
             raise ValueError("the adjoint workload needs SolverConfig(perm_field=True)")
         if cfg.precision not in ("f32", "f64"):
             raise ValueError("the adjoint runs with f32 or f64 populations")
+        self.wl = wl
+        self.ctx = HipContext(cfg, wl.geometry.flags, compact=compact)
+        if checkpoint == "auto":
+            state_bytes = self.ctx.f[0].numel() * self.ctx.f[0].element_size()
+            free, _ = torch.cuda.mem_get_info(self.ctx.device)
+            checkpoint = auto_checkpoint(steps, state_bytes, free)
         self.store_all = checkpoint is None
         if self.store_all:
             checkpoint = steps
         if steps % checkpoint:
             raise ValueError("steps must be a multiple of the checkpoint interval")
-        self.wl = wl
         self.steps, self.window, self.checkpoint = steps, window, checkpoint
-        self.ctx = HipContext(cfg, wl.geometry.flags, compact=compact)
         self.ctx.perm.copy_(torch.as_tensor(K, dtype=cfg.real_dtype))
         shape = self.ctx.shape
         rd = cfg.real_dtype

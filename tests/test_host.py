@@ -17,6 +17,7 @@ from pourover_amd import backends as B
 from pourover_amd import fields as F
 from pourover_amd import geometry as G
 from pourover_amd import workloads as WL
+from pourover_amd.adjoint import auto_checkpoint
 from pourover_amd.engine import CHUNK_CELLS, WRITE_GRANULE, HipContext, SolverConfig
 from pourover_amd.solver import HipLBMSolver, LBMSolverProtocol, ParticleCoupledLBMProtocol
 
@@ -174,3 +175,15 @@ def test_bench_refuses_more_than_one_gpu():
                        stderr=subprocess.PIPE, text=True, timeout=120)
     assert p.returncode != 0
     assert "one GPU" in p.stderr
+
+
+# ---- adjoint state store ------------------------------------------------------------
+def test_auto_checkpoint_keeps_states_that_fit():
+    mb = 1 << 20
+    # 500 states of 56 MB fit in half of 200 GB: keep them all
+    assert auto_checkpoint(500, 56 * mb, 200 * 1024 * mb) is None
+    # otherwise the largest divisor of steps not above 50
+    assert auto_checkpoint(500, 56 * mb, 20 * 1024 * mb) == 50
+    assert auto_checkpoint(120, 56 * mb, 1024 * mb) == 40
+    assert auto_checkpoint(97, 56 * mb, 1024 * mb) == 1
+    assert auto_checkpoint(30, 56 * mb, 1024 * mb) == 30
