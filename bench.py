@@ -282,6 +282,57 @@ def bench_adjoint(n=128, steps=100, reps=20):
     return out
 
 
+def bench_diagnostics(reps=20, host=True):
+    """Field diagnostics at 224^3 (V60, FP32): one pol_field_stats call (the
+    statistics every reference diagnostic reads: rho / u / flags pass, FP64
+    block partials, 43 doubles to the host) timed per call, against the
+    reference's host path -- rho and u copied to the host, then the numpy
+    diagnostics (oracle/diagnostics.py: quality, conservation, flow, V60
+    physics) -- timed once."""
+    import numpy as np
+    import torch
+
+    from pourover_amd import diagnostics as D
+    from pourover_amd import workloads as WL
+    from pourover_amd.engine import HipContext
+
+    wl = WL.v60_config(224, "f32")
+    ctx = HipContext(wl.config, wl.geometry.flags)
+    ctx.init_equilibrium()
+    ctx.step(20)
+    ctx.macroscopic(macro=True)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        D.field_statistics(ctx)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        D.field_statistics(ctx)
+    call_ms = (time.perf_counter() - t0) / reps * 1e3
+    n = wl.config.cells
+    algo = 20 * n  # rho 4 B + u 12 B + flags 4 B per cell
+    out = {"grid": [224, 224, 224], "ms_per_call": round(call_ms, 4), "algorithmic_bytes": algo,
+           "achieved_GBs_wall": round(algo / call_ms / 1e6, 1), "frac_wall": round(algo / call_ms / 1e6 / HBM_PEAK_GBS, 4)}
+    if host:
+        sys.path.insert(0, ROOT)
+        from oracle import diagnostics as OD
+
+        t0 = time.perf_counter()
+        rho = ctx.rho.cpu().numpy().transpose(2, 1, 0)
+        u = ctx.u.cpu().numpy().transpose(3, 2, 1, 0)
+        t1 = time.perf_counter()
+        OD.lbm_quality(rho, u)
+        OD.conservation(rho, u, {"initial_mass": 0.0, "accumulated_inflow": 0.0, "prev_momentum": np.zeros(3)}, 0.0)
+        OD.flow_analysis(u)
+        OD.v60_physics(rho, u, 0.14 / 224, 0.01)
+        t2 = time.perf_counter()
+        out["host_path_ms"] = round((t2 - t0) * 1e3, 1)
+        out["host_copy_ms"] = round((t1 - t0) * 1e3, 1)
+        out["host_path_note"] = "rho / u to host + numpy diagnostics (the reference's to_numpy path), one core"
+    del ctx
+    torch.cuda.empty_cache()
+    return out
+
+
 def counter_traffic(precision):
     """HBM bytes per launch of the stream-collide kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
@@ -306,6 +357,8 @@ def main():
                     help="only the two-way coupled 224^3 step with 50 000 particles (for profiling)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the 128^3 adjoint measurement (for profiling)")
+    ap.add_argument("--diagnostics-only", action="store_true",
+                    help="only the 224^3 field-diagnostics measurement (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
@@ -328,6 +381,10 @@ def main():
     if args.adjoint_only:
         log("bench: adjoint 128^3 f32")
         print(json.dumps({"adjoint": bench_adjoint()}), flush=True)
+        return
+    if args.diagnostics_only:
+        log("bench: field diagnostics 224^3 f32")
+        print(json.dumps({"diagnostics_224_f32": bench_diagnostics(host=False)}), flush=True)
         return
     if args.calibrate_copy:
         out = copy_calibration()
@@ -383,6 +440,8 @@ def main():
         out["coupled_224_f32_50k_particles"] = bench_coupled("f32", args.steps, args.warmup)
         log("bench: adjoint 128^3 f32")
         out["adjoint"] = bench_adjoint()
+        log("bench: field diagnostics 224^3 f32")
+        out["diagnostics_224_f32"] = bench_diagnostics()
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

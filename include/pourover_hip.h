@@ -308,6 +308,84 @@ int pol_adjoint_reset(pol_ctx *ctx, void *stream);
  * synchronous. */
 int pol_adjoint_collect(pol_ctx *ctx, double *out4, void *stream);
 
+/* ---- field diagnostics (src/core/numerical_stability.py,
+ *      src/visualization/lbm_diagnostics.py, src/visualization/visualizer.py) --
+ * The reference reduces host copies of rho, u and f in numpy / Taichi loops
+ * (numerical_stability.py:53-110, lbm_diagnostics.py:188-460,
+ * visualizer.py:120-167).  Here one device pass over the bound rho / u
+ * buffers (and optionally the populations) produces every sum, count and
+ * extreme those consumers read, in FP64 and in a fixed block order
+ * (deterministic).  Magnitudes |u| are formed in FP32 as
+ * sqrt((ux*ux + uy*uy) + uz*uz) without contraction, the per-element value
+ * the reference's float32 numpy expressions give. */
+#define POL_STAT_CELLS 0          /* all cells */
+#define POL_STAT_SUM_DRHO 1       /* sum (rho - 1) */
+#define POL_STAT_SUM_DRHO2 2      /* sum (rho - 1)^2 */
+#define POL_STAT_MAX_ABS_DRHO 3   /* max |rho - 1| (NaN if any rho is NaN) */
+#define POL_STAT_MOM_X 4          /* sum rho u */
+#define POL_STAT_MOM_Y 5
+#define POL_STAT_MOM_Z 6
+#define POL_STAT_SUM_UMAG 7       /* sum |u| */
+#define POL_STAT_SUM_UMAG2 8      /* sum |u|^2 */
+#define POL_STAT_MAX_UMAG 9       /* max |u| (NaN if any |u| is NaN) */
+#define POL_STAT_SUM_MACH_ACTIVE 10 /* sum of Mach = |u| / c_s (FP32) over Mach > 1e-10 */
+#define POL_STAT_ACTIVE 11        /* cells with Mach > 1e-10 */
+#define POL_STAT_MACH_GT 12       /* cells with Mach > 0.1 */
+#define POL_STAT_SUM_UX 13
+#define POL_STAT_SUM_UY 14
+#define POL_STAT_SUM_UZ 15
+#define POL_STAT_SUM_UX2 16
+#define POL_STAT_SUM_UY2 17
+#define POL_STAT_SUM_UZ2 18
+#define POL_STAT_SUM_UMAG_TOP 19  /* sum |u| over z >= z_top */
+#define POL_STAT_SUM_UMAG_MID 20  /* z_mid <= z < z_top */
+#define POL_STAT_SUM_UMAG_BOT 21  /* z < z_mid */
+#define POL_STAT_OUTLET_DOWN 22   /* sum over z < z_outlet of max(-u_z, 0) */
+#define POL_STAT_DRHO_PREV2 23    /* sum (rho - prev_rho)^2 (prev_rho given) */
+#define POL_STAT_DU_PREV2 24      /* sum |u - prev_u|^2 (prev_u given) */
+#define POL_STAT_FLUID 25         /* non-solid cells */
+#define POL_STAT_FLUID_RHO_NAN 26
+#define POL_STAT_FLUID_RHO_INF 27
+#define POL_STAT_FLUID_RHO_MIN 28 /* over finite rho; 1e300 when none */
+#define POL_STAT_FLUID_RHO_MAX 29 /* over finite rho; -1e300 when none */
+#define POL_STAT_FLUID_U_NAN 30   /* |u| NaN */
+#define POL_STAT_FLUID_U_INF 31   /* |u| infinite */
+#define POL_STAT_FLUID_UMAX 32    /* over finite |u|; 0 when none */
+#define POL_STAT_FLUID_POP_NAN 33 /* NaN populations (with_populations) */
+#define POL_STAT_FLUID_POP_INF 34 /* infinite populations (with_populations) */
+#define POL_STAT_INT_CELLS 35     /* non-solid cells with 1 <= x, y, z <= n - 2 */
+#define POL_STAT_INT_WATER 36     /* sum rho over those with phase > 0.1 */
+#define POL_STAT_INT_AIR 37       /* sum rho over the others */
+#define POL_STAT_INT_UMAX 38      /* max |u| over interior non-solid cells; -1e300 when none */
+#define POL_STAT_INT_UMIN 39      /* min |u|; 1e300 when none */
+#define POL_STAT_INT_USUM 40      /* sum |u| */
+#define POL_STAT_SUM_UMAG_ACTIVE 41 /* sum |u| over |u| > 1e-10 */
+#define POL_STAT_UMAG_ACTIVE 42   /* cells with |u| > 1e-10 */
+#define POL_NSTAT 43
+
+typedef struct pol_stats_opts {
+  const void *prev_rho;    /* real [cells] previous density, or NULL */
+  const void *prev_u;      /* real [3][cells] previous velocity, or NULL */
+  const void *phase;       /* real [cells] phase field, or NULL (every cell water) */
+  int32_t z_top, z_mid;    /* band edges of the regional |u| sums */
+  int32_t z_outlet;        /* outlet slab: z < z_outlet */
+  int32_t with_populations; /* also count NaN / Inf populations of the current state */
+} pol_stats_opts;
+/* Statistics of the bound rho / u buffers (compute them first, e.g. with
+ * pol_macroscopic) into out[POL_NSTAT] (host) and, when z_profile is not
+ * NULL, sum_{x,y} rho per z plane into z_profile[nz] (host).  Synchronous. */
+int pol_field_stats(pol_ctx *ctx, const pol_stats_opts *opts, double *out, double *z_profile, void *stream);
+/* magnitude[cell] = |u| (FP32 form above), pressure[cell] = rho * (float)(1/3)
+ * (the export fields of enhanced_visualizer.py:1487-1505); real [cells]
+ * device buffers. */
+int pol_export_fields(pol_ctx *ctx, void *magnitude, void *pressure, void *stream);
+/* Emergency stabilisation of the non-solid cells
+ * (numerical_stability.py:165-197): rho NaN / Inf / <= 0 -> 1, else clamped
+ * into [min_density, max_density]; |u| NaN / Inf -> u = 0, |u| > max_velocity
+ * -> scaled to max_velocity; every NaN / Inf population of the current state
+ * -> the equilibrium of the repaired (rho, u). */
+int pol_stabilize(pol_ctx *ctx, double max_velocity, double min_density, double max_density, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

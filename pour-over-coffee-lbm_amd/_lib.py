@@ -47,6 +47,53 @@ POL_WANT_REDUCE = 0x2
 RED_SLOTS = ("mass", "mom_x", "mom_y", "mom_z", "kinetic_energy", "max_velocity", "nonfinite_cells",
              "outlet_flux", "cells", "rho_min", "rho_max", "inlet_flux")
 
+# field statistics slots (pol_field_stats)
+POL_STAT_CELLS = 0
+POL_STAT_SUM_DRHO = 1
+POL_STAT_SUM_DRHO2 = 2
+POL_STAT_MAX_ABS_DRHO = 3
+POL_STAT_MOM_X = 4
+POL_STAT_MOM_Y = 5
+POL_STAT_MOM_Z = 6
+POL_STAT_SUM_UMAG = 7
+POL_STAT_SUM_UMAG2 = 8
+POL_STAT_MAX_UMAG = 9
+POL_STAT_SUM_MACH_ACTIVE = 10
+POL_STAT_ACTIVE = 11
+POL_STAT_MACH_GT = 12
+POL_STAT_SUM_UX = 13
+POL_STAT_SUM_UY = 14
+POL_STAT_SUM_UZ = 15
+POL_STAT_SUM_UX2 = 16
+POL_STAT_SUM_UY2 = 17
+POL_STAT_SUM_UZ2 = 18
+POL_STAT_SUM_UMAG_TOP = 19
+POL_STAT_SUM_UMAG_MID = 20
+POL_STAT_SUM_UMAG_BOT = 21
+POL_STAT_OUTLET_DOWN = 22
+POL_STAT_DRHO_PREV2 = 23
+POL_STAT_DU_PREV2 = 24
+POL_STAT_FLUID = 25
+POL_STAT_FLUID_RHO_NAN = 26
+POL_STAT_FLUID_RHO_INF = 27
+POL_STAT_FLUID_RHO_MIN = 28
+POL_STAT_FLUID_RHO_MAX = 29
+POL_STAT_FLUID_U_NAN = 30
+POL_STAT_FLUID_U_INF = 31
+POL_STAT_FLUID_UMAX = 32
+POL_STAT_FLUID_POP_NAN = 33
+POL_STAT_FLUID_POP_INF = 34
+POL_STAT_INT_CELLS = 35
+POL_STAT_INT_WATER = 36
+POL_STAT_INT_AIR = 37
+POL_STAT_INT_UMAX = 38
+POL_STAT_INT_UMIN = 39
+POL_STAT_INT_USUM = 40
+POL_STAT_SUM_UMAG_ACTIVE = 41
+POL_STAT_UMAG_ACTIVE = 42
+POL_NSTAT = 43
+STAT_SLOTS = ("cells", "sum_drho", "sum_drho2", "max_abs_drho", "mom_x", "mom_y", "mom_z", "sum_umag", "sum_umag2", "max_umag", "sum_mach_active", "active", "mach_gt", "sum_ux", "sum_uy", "sum_uz", "sum_ux2", "sum_uy2", "sum_uz2", "sum_umag_top", "sum_umag_mid", "sum_umag_bot", "outlet_down", "drho_prev2", "du_prev2", "fluid", "fluid_rho_nan", "fluid_rho_inf", "fluid_rho_min", "fluid_rho_max", "fluid_u_nan", "fluid_u_inf", "fluid_umax", "fluid_pop_nan", "fluid_pop_inf", "int_cells", "int_water", "int_air", "int_umax", "int_umin", "int_usum", "sum_umag_active", "umag_active")
+
 STORAGE_CODES = {"f32": POL_F32, "f16": POL_F16, "bf16": POL_BF16, "f64": POL_F64}
 
 
@@ -85,6 +132,11 @@ class PolParticleBuffers(C.Structure):
         "drag_field", "u_field")]
 
 
+class PolStatsOpts(C.Structure):
+    _fields_ = [("prev_rho", C.c_void_p), ("prev_u", C.c_void_p), ("phase", C.c_void_p), ("z_top", C.c_int32),
+                ("z_mid", C.c_int32), ("z_outlet", C.c_int32), ("with_populations", C.c_int32)]
+
+
 class PolAdjointBuffers(C.Structure):
     _fields_ = [("lam", C.c_void_p * 2), ("grad_perm", C.c_void_p), ("rev_flags", C.c_void_p)]
 
@@ -117,6 +169,9 @@ SIGNATURES = {
     "pol_adjoint_step": (C.c_int, [_P, _P, C.c_int, _P]),
     "pol_adjoint_reset": (C.c_int, [_P, _P]),
     "pol_adjoint_collect": (C.c_int, [_P, _P, _P]),
+    "pol_field_stats": (C.c_int, [_P, C.POINTER(PolStatsOpts), _P, _P, _P]),
+    "pol_export_fields": (C.c_int, [_P, _P, _P, _P]),
+    "pol_stabilize": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, _P]),
 }
 
 _lib = None

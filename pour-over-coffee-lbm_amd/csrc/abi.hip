@@ -1,5 +1,6 @@
 // C ABI of libpourover_hip.so (declared in include/pourover_hip.h).
 // Host-side only: validation, context state, launch dispatch.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -32,6 +33,11 @@ struct pol_ctx {
   int acur = 0;
   double* adj_partial = nullptr;  // [nblk][4]
   double* adj_acc = nullptr;      // [4]: dJ/du_in (3), J
+  // field statistics
+  uint32_t stat_tiles = 0;          // blocks per z plane
+  double* stat_partial = nullptr;   // [nz * stat_tiles][POL_NSTAT]
+  double* stat_zpart = nullptr;     // [nz * stat_tiles]
+  double* stat_dev = nullptr;       // [POL_NSTAT + nz]
   std::string err;
 };
 
@@ -108,11 +114,22 @@ int pol_create(pol_ctx** out, const pol_desc* desc) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->red_dev, POL_NRED * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->adj_partial, (size_t)c->part_cap * 4 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->adj_acc, 4 * sizeof(double));
+  // statistics: up to 8 blocks per plane (1792 blocks at 224^3, ~24 cells per
+  // thread: the per-block reduction of the 43 statistics is amortised)
+  const uint64_t nxy = (uint64_t)d.nx * d.ny;
+  c->stat_tiles = (uint32_t)std::min<uint64_t>((nxy + BLOCK - 1) / BLOCK, 8);
+  const size_t sblk = (size_t)c->stat_tiles * d.nz;
+  if (e == hipSuccess) e = hipMalloc((void**)&c->stat_partial, sblk * POL_NSTAT * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->stat_zpart, sblk * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->stat_dev, ((size_t)POL_NSTAT + d.nz) * sizeof(double));
   if (e != hipSuccess) {
     (void)hipFree(c->partial);
     (void)hipFree(c->red_dev);
     (void)hipFree(c->adj_partial);
     (void)hipFree(c->adj_acc);
+    (void)hipFree(c->stat_partial);
+    (void)hipFree(c->stat_zpart);
+    (void)hipFree(c->stat_dev);
     delete c;
     return POL_ERR_ALLOC;
   }
@@ -127,6 +144,9 @@ int pol_destroy(pol_ctx* c) {
   (void)hipFree(c->red_dev);
   (void)hipFree(c->adj_partial);
   (void)hipFree(c->adj_acc);
+  (void)hipFree(c->stat_partial);
+  (void)hipFree(c->stat_zpart);
+  (void)hipFree(c->stat_dev);
   delete c;
   return POL_OK;
 }
@@ -493,6 +513,70 @@ int pol_adjoint_collect(pol_ctx* c, double* out4, void* stream) {
   if (e == hipSuccess) e = hipMemcpyAsync(out4, c->adj_acc, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_adjoint_collect");
+}
+
+// ---- field diagnostics --------------------------------------------------------
+int pol_field_stats(pol_ctx* c, const pol_stats_opts* o, double* out, double* z_profile, void* stream) {
+  if (!c || !o || !out) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_field_stats: buffers not bound");
+  if (!c->b.rho || !c->b.u) return fail(c, POL_ERR_INVALID, "pol_field_stats: needs the rho / u buffers");
+  hipStream_t st = (hipStream_t)stream;
+  const pol_desc& d = c->d;
+  StatsArgs s{};
+  s.rho = c->b.rho;
+  s.u = c->b.u;
+  s.flags = (d.options & POL_USE_FLAGS) ? c->b.cell_flags : nullptr;
+  s.prev_rho = o->prev_rho;
+  s.prev_u = o->prev_u;
+  s.phase = o->phase;
+  s.nx = d.nx;
+  s.ny = d.ny;
+  s.nz = d.nz;
+  s.nxy = (uint32_t)d.nx * d.ny;
+  s.N = c->N;
+  s.tiles = c->stat_tiles;
+  s.z_top = o->z_top;
+  s.z_mid = o->z_mid;
+  s.z_out = o->z_outlet;
+  s.partial = c->stat_partial;
+  s.zpart = c->stat_zpart;
+  hipError_t e = hipSetDevice(d.device);
+  if (e == hipSuccess) e = launch_field_stats(d.storage == POL_F64, s, c->stat_dev, c->stat_dev + POL_NSTAT, st);
+  if (e == hipSuccess && o->with_populations) {
+    const StepArgs a = make_args(c, c->b.f[c->cur], nullptr, 0u);
+    e = launch_pop_nonfinite(d.storage, a, c->partial, c->stat_dev, c->wgrid, st);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, c->stat_dev, POL_NSTAT * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && z_profile)
+    e = hipMemcpyAsync(z_profile, c->stat_dev + POL_NSTAT, (size_t)d.nz * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_field_stats");
+}
+
+int pol_export_fields(pol_ctx* c, void* magnitude, void* pressure, void* stream) {
+  if (!c || !magnitude || !pressure) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_export_fields: buffers not bound");
+  if (!c->b.rho || !c->b.u) return fail(c, POL_ERR_INVALID, "pol_export_fields: needs the rho / u buffers");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_export_fields(c->d.storage == POL_F64, c->b.rho, c->b.u, c->N, magnitude, pressure,
+                             (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_export_fields");
+}
+
+int pol_stabilize(pol_ctx* c, double max_velocity, double min_density, double max_density, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_stabilize: buffers not bound");
+  if (!c->b.rho || !c->b.u) return fail(c, POL_ERR_INVALID, "pol_stabilize: needs the rho / u buffers");
+  if (!(max_velocity > 0.0) || !(min_density > 0.0) || !(max_density >= min_density))
+    return fail(c, POL_ERR_INVALID, "pol_stabilize: thresholds must satisfy 0 < min_density <= max_density, "
+                                    "max_velocity > 0");
+  const StepArgs a = make_args(c, c->b.f[c->cur], c->b.f[c->cur], 0u);
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_stabilize(c->d.storage, a, (float)max_velocity, (float)min_density, (float)max_density, c->wgrid,
+                         (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_stabilize");
 }
 
 }  // extern "C"

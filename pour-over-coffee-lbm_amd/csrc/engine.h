@@ -57,6 +57,35 @@ hipError_t launch_reduce_final(const double* partial, uint32_t nblk, double* out
 hipError_t launch_init_equilibrium(int storage, const StepArgs& a, const void* rho, const void* u, const Grid& g,
                                    hipStream_t s);
 
+// Field statistics (diagnostics.hip): one pass over rho / u (+ flags, the
+// previous rho / u and a phase field when given) on a (tiles, nz) grid.
+struct StatsArgs {
+  const void* rho;
+  const void* u;
+  const uint32_t* flags;  // may be null (no solids)
+  const void* prev_rho;   // may be null
+  const void* prev_u;     // may be null
+  const void* phase;      // may be null (every cell water)
+  uint32_t nx, ny, nz, nxy, N;
+  uint32_t tiles;         // blocks per z plane
+  int32_t z_top, z_mid, z_out;
+  double* partial;        // [nz * tiles][POL_NSTAT]
+  double* zpart;          // [nz * tiles]
+};
+// Statistics into out_dev[POL_NSTAT] and the per-plane density sums into
+// zout_dev[nz] (device), deterministic block order.
+hipError_t launch_field_stats(bool real_double, const StatsArgs& s, double* out_dev, double* zout_dev, hipStream_t st);
+// NaN / Inf populations of a.fin over the non-solid cells of the launch list
+// into out_dev[POL_STAT_FLUID_POP_NAN], [POL_STAT_FLUID_POP_INF].
+hipError_t launch_pop_nonfinite(int storage, const StepArgs& a, double* partial, double* out_dev, const Grid& g,
+                                hipStream_t st);
+// Emergency stabilisation of rho / u and of the populations in a.fout.
+hipError_t launch_stabilize(int storage, const StepArgs& a, float max_u, float min_rho, float max_rho, const Grid& g,
+                            hipStream_t st);
+// |u| and pressure = rho / 3 fields (real [N]).
+hipError_t launch_export_fields(bool real_double, const void* rho, const void* u, uint32_t N, void* mag, void* pres,
+                                hipStream_t st);
+
 // Particle state in the real type (float, or double with real_is_double).
 struct ParticleArgs {
   int32_t count;

@@ -113,12 +113,15 @@ int main(void) {
   F(pol_particle_buffers, u_field)
   S(pol_adjoint_buffers) F(pol_adjoint_buffers, lam) F(pol_adjoint_buffers, grad_perm)
   F(pol_adjoint_buffers, rev_flags)
+  S(pol_stats_opts) F(pol_stats_opts, prev_rho) F(pol_stats_opts, prev_u) F(pol_stats_opts, phase)
+  F(pol_stats_opts, z_top) F(pol_stats_opts, z_mid) F(pol_stats_opts, z_outlet) F(pol_stats_opts, with_populations)
   return 0;
 }
 """
 
 _CTYPES = {"pol_desc": L.PolDesc, "pol_buffers": L.PolBuffers, "pol_particle_desc": L.PolParticleDesc,
-           "pol_particle_buffers": L.PolParticleBuffers, "pol_adjoint_buffers": L.PolAdjointBuffers}
+           "pol_particle_buffers": L.PolParticleBuffers, "pol_adjoint_buffers": L.PolAdjointBuffers,
+           "pol_stats_opts": L.PolStatsOpts}
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
