@@ -11,11 +11,12 @@ no threads except the solid cells inside a row's run.
 
 Secondary lines in the same JSON object: the all-fluid periodic 224^3 box
 (every cell updated), FP16/BF16 storage speed-up over FP32, the two-way
-coupled step with 50 000 particles, adjoint cost per forward step, and the
-CPU baseline (C restatements on the host cores, bounded samples).
+coupled step with 50 000 particles, adjoint cost per forward step, field
+diagnostics per call, the thermal D3Q7 step, and the CPU baseline (C
+restatements on the host cores, bounded samples).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--precision P]
-       [--coupled-only | --adjoint-only] [--calibrate-copy]
+       [--coupled-only | --adjoint-only | --diagnostics-only | --thermal-only] [--calibrate-copy]
 """
 from __future__ import annotations
 
@@ -333,6 +334,72 @@ def bench_diagnostics(reps=20, host=True):
     return out
 
 
+# thermal D3Q7 algorithmic bytes per cell and step (FP32): collide reads g
+# (7 x 4) and the heat source (4) and writes g (7 x 4); the temperature pass
+# reads g (7 x 4) and writes T (4).  Convection adds T (4, the stencil counted
+# once), u (3 x 4) and the heat-source write (4) to the collide pass (the
+# base heat source is read in place of the heat source).
+THERMAL_BYTES = {False: 92, True: 112}
+
+
+def bench_thermal(n=224, steps=200, warmup=20, cpu=True):
+    """Thermal D3Q7 step at n^3, FP32 (thermal_lbm.py:396-427 without the
+    per-step flux pass): pure diffusion, and with the -u.grad T convection
+    source in the coupled loop's form (each step starts from the base heat
+    source, thermal_fluid_coupled.py:193-199).  Timed with HIP events on the stream the kernels run on; the
+    numpy restatement (oracle/thermal.py, one core) on a 128^3 sample is the
+    CPU baseline."""
+    import numpy as np
+    import torch
+
+    from pourover_amd import thermal as TH
+
+    th = TH.ThermalLBM(grid=(n, n, n), precision="f32")
+    th.complete_initialization(25.0, 93.0, n // 4)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    th._vel.copy_((torch.rand(th._vel.shape, generator=g, device="cuda") - 0.5) * 0.02)
+    out = {"grid": [n, n, n], "omega": round(th.omega_thermal, 6), "steps": steps}
+    for conv in (False, True):
+        th.enable_convection_coupling(conv)
+        if conv:  # the coupled loop's form: each step starts from the base heat source
+            th.use_base_heat_source(torch.zeros_like(th._hs))
+        th.run(warmup)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        ok = th.run(steps)
+        stop.record()
+        torch.cuda.synchronize()
+        sec = start.elapsed_time(stop) / 1e3 / steps
+        algo = THERMAL_BYTES[conv] * n ** 3
+        out["convection" if conv else "diffusion"] = {
+            "ms_per_step": round(sec * 1e3, 4), "mlups": round(n ** 3 / sec / 1e6, 1),
+            "algorithmic_bytes_per_step": algo, "achieved_GBs": round(algo / sec / 1e9, 1),
+            "frac": round(algo / sec / 1e9 / HBM_PEAK_GBS, 4), "stable": bool(ok)}
+    del th
+    torch.cuda.empty_cache()
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import thermal as OT
+
+        m = 128
+        rng = np.random.default_rng(1)
+        T = rng.uniform(20, 90, (m, m, m)).astype(np.float32)
+        st = {"g": OT.equilibrium(T), "T": T, "heat_source": np.zeros_like(T),
+              "u": (rng.random((m, m, m, 3), dtype=np.float32) - 0.5) * 0.02}
+        ns, t0 = 0, time.perf_counter()
+        while True:
+            st = OT.step(st, 1.0 / 0.577, convection=True)
+            ns += 1
+            el = time.perf_counter() - t0
+            if el > 8.0 or ns >= 30:
+                break
+        out["cpu_baseline"] = {"value": round(m ** 3 * ns / el / 1e6, 2), "unit": "MLUPS", "cores": 1, "kind": "port",
+                               "sample": "%d convection steps of a %d^3 box, numpy restatement of thermal_lbm.py, "
+                                         "one core" % (ns, m)}
+    return out
+
+
 def counter_traffic(precision):
     """HBM bytes per launch of the stream-collide kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
@@ -359,6 +426,8 @@ def main():
                     help="only the 128^3 adjoint measurement (for profiling)")
     ap.add_argument("--diagnostics-only", action="store_true",
                     help="only the 224^3 field-diagnostics measurement (for profiling)")
+    ap.add_argument("--thermal-only", action="store_true",
+                    help="only the 224^3 thermal D3Q7 measurement (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
@@ -374,6 +443,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     prec = args.precision
+    if args.calibrate_copy:
+        out = copy_calibration()
+        log("bench: copy calibration %s" % json.dumps(out))
     if args.coupled_only:
         log("bench: coupled 224^3 %s + 50k particles" % prec)
         print(json.dumps({"coupled_224_50k_particles": bench_coupled(prec, args.steps, args.warmup)}), flush=True)
@@ -386,9 +458,11 @@ def main():
         log("bench: field diagnostics 224^3 f32")
         print(json.dumps({"diagnostics_224_f32": bench_diagnostics(host=False)}), flush=True)
         return
-    if args.calibrate_copy:
-        out = copy_calibration()
-        log("bench: copy calibration %s" % json.dumps(out))
+    if args.thermal_only:
+        log("bench: thermal D3Q7 224^3 f32")
+        print(json.dumps({"thermal_224_f32": bench_thermal(steps=args.steps, warmup=args.warmup, cpu=False)}),
+              flush=True)
+        return
     log("bench: v60 224^3 %s, %d warmup + %d timed steps" % (prec, args.warmup, args.steps))
     r = bench_v60(prec, args.steps, args.warmup)
     k_s = r["event_s"] / args.steps
@@ -442,6 +516,8 @@ def main():
         out["adjoint"] = bench_adjoint()
         log("bench: field diagnostics 224^3 f32")
         out["diagnostics_224_f32"] = bench_diagnostics()
+        log("bench: thermal D3Q7 224^3 f32")
+        out["thermal_224_f32"] = bench_thermal(steps=args.steps, warmup=args.warmup)
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

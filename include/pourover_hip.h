@@ -386,6 +386,60 @@ int pol_export_fields(pol_ctx *ctx, void *magnitude, void *pressure, void *strea
  * -> the equilibrium of the repaired (rho, u). */
 int pol_stabilize(pol_ctx *ctx, double max_velocity, double min_density, double max_density, void *stream);
 
+/* ---- thermal D3Q7 (src/physics/thermal_lbm.py) -----------------------------
+ * Temperature populations g_q (q = rest, +x, -x, +y, -y, +z, -z; weights 1/4,
+ * 1/8 x 6) over every cell of the box, stored after streaming like the
+ * reference's `g`, equilibrium w_q T.  One step (thermal_lbm.py:396-427):
+ *   T = sum_q g_q; S = heat_source (+ -u.grad T by central differences on
+ *   interior cells, added into heat_source, with POL_THERMAL_CONVECTION);
+ *   g*_q = (g_q - omega (g_q - w_q T)) + w_q S; streamed to x + e_q, a
+ *   population whose source x - e_q lies outside the box keeping the cell's
+ *   own g*_q (thermal_lbm.py:231-253); T' = sum_q g'_q.
+ * FP32 arithmetic in the reference's operation order without contraction
+ * (FP64 for POL_F64 contexts). */
+#define POL_THERMAL_CONVECTION 0x1u
+/* Every step starts from heat_source_base: S = base (+ convection), written
+ * to heat_source -- the coupled loop's reset_heat_source_to_base before each
+ * thermal step (thermal_fluid_coupled.py:193-194, one sub-cycle) fused into
+ * the step. */
+#define POL_THERMAL_RESET_SOURCE 0x2u
+
+typedef struct pol_thermal_desc {
+  double omega;        /* 1 / tau_thermal */
+  uint32_t options;    /* POL_THERMAL_CONVECTION | POL_THERMAL_RESET_SOURCE */
+  int32_t reserved;
+} pol_thermal_desc;
+
+typedef struct pol_thermal_buffers {
+  void *g[2];            /* real [7][cells] streamed populations, ping-pong */
+  void *temperature[2];  /* real [cells] T after the step / before it, ping-pong */
+  void *heat_source;     /* real [cells] S (convection adds -u.grad T into it) */
+  const void *velocity;  /* real [3][cells] advecting velocity, or NULL */
+  const void *conductivity; /* real [cells] k of the heat flux, or NULL (0.68) */
+  const void *heat_source_base; /* real [cells] base S for POL_THERMAL_RESET_SOURCE, or NULL */
+} pol_thermal_buffers;
+/* Attach the thermal buffers (g[0] / temperature[0] current). */
+int pol_thermal_bind(pol_ctx *ctx, const pol_thermal_desc *d, const pol_thermal_buffers *b);
+/* Change omega / options of a bound thermal state. */
+int pol_thermal_set(pol_ctx *ctx, const pol_thermal_desc *d);
+/* g_q = w_q T from the current temperature (thermal_lbm.py:132-170). */
+int pol_thermal_init(pol_ctx *ctx, void *stream);
+/* nsteps thermal steps.  With unstable_out != NULL the last step's stability
+ * test (thermal_lbm.py:348-376: T outside [-50, 150] or NaN, a population NaN
+ * or |g| > 1000) is copied to *unstable_out (count of failing cells);
+ * synchronous then. */
+int pol_thermal_step(pol_ctx *ctx, int nsteps, int32_t *unstable_out, void *stream);
+/* Heat flux q = -k grad T (central differences, interior cells; 0 on the
+ * faces) of the current temperature into flux (real [3][cells];
+ * thermal_lbm.py:268-286). */
+int pol_thermal_heat_flux(pol_ctx *ctx, void *flux, void *stream);
+/* {min, max, sum} of the current temperature (FP64, host); synchronous. */
+int pol_thermal_stats(pol_ctx *ctx, double *out3, void *stream);
+/* Boundary forms of thermal_lbm.py:288-346 on the cells where mask (uint8
+ * [cells]) is non-zero: kind 1 Dirichlet (T = a, g = w a), kind 2 Neumann
+ * (S = a), kind 3 convective (S = -a (T - b)). */
+int pol_thermal_boundary(pol_ctx *ctx, int kind, const uint8_t *mask, double a, double b, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,9 @@ POL_STAT_UMAG_ACTIVE = 42
 POL_NSTAT = 43
 STAT_SLOTS = ("cells", "sum_drho", "sum_drho2", "max_abs_drho", "mom_x", "mom_y", "mom_z", "sum_umag", "sum_umag2", "max_umag", "sum_mach_active", "active", "mach_gt", "sum_ux", "sum_uy", "sum_uz", "sum_ux2", "sum_uy2", "sum_uz2", "sum_umag_top", "sum_umag_mid", "sum_umag_bot", "outlet_down", "drho_prev2", "du_prev2", "fluid", "fluid_rho_nan", "fluid_rho_inf", "fluid_rho_min", "fluid_rho_max", "fluid_u_nan", "fluid_u_inf", "fluid_umax", "fluid_pop_nan", "fluid_pop_inf", "int_cells", "int_water", "int_air", "int_umax", "int_umin", "int_usum", "sum_umag_active", "umag_active")
 
+POL_THERMAL_CONVECTION = 0x1
+POL_THERMAL_RESET_SOURCE = 0x2
+
 STORAGE_CODES = {"f32": POL_F32, "f16": POL_F16, "bf16": POL_BF16, "f64": POL_F64}
 
 
@@ -137,6 +140,15 @@ class PolStatsOpts(C.Structure):
                 ("z_mid", C.c_int32), ("z_outlet", C.c_int32), ("with_populations", C.c_int32)]
 
 
+class PolThermalDesc(C.Structure):
+    _fields_ = [("omega", C.c_double), ("options", C.c_uint32), ("reserved", C.c_int32)]
+
+
+class PolThermalBuffers(C.Structure):
+    _fields_ = [("g", C.c_void_p * 2), ("temperature", C.c_void_p * 2), ("heat_source", C.c_void_p),
+                ("velocity", C.c_void_p), ("conductivity", C.c_void_p), ("heat_source_base", C.c_void_p)]
+
+
 class PolAdjointBuffers(C.Structure):
     _fields_ = [("lam", C.c_void_p * 2), ("grad_perm", C.c_void_p), ("rev_flags", C.c_void_p)]
 
@@ -172,6 +184,13 @@ SIGNATURES = {
     "pol_field_stats": (C.c_int, [_P, C.POINTER(PolStatsOpts), _P, _P, _P]),
     "pol_export_fields": (C.c_int, [_P, _P, _P, _P]),
     "pol_stabilize": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, _P]),
+    "pol_thermal_bind": (C.c_int, [_P, C.POINTER(PolThermalDesc), C.POINTER(PolThermalBuffers)]),
+    "pol_thermal_set": (C.c_int, [_P, C.POINTER(PolThermalDesc)]),
+    "pol_thermal_init": (C.c_int, [_P, _P]),
+    "pol_thermal_step": (C.c_int, [_P, C.c_int, _P, _P]),
+    "pol_thermal_heat_flux": (C.c_int, [_P, _P, _P]),
+    "pol_thermal_stats": (C.c_int, [_P, _P, _P]),
+    "pol_thermal_boundary": (C.c_int, [_P, C.c_int, _P, C.c_double, C.c_double, _P]),
 }
 
 _lib = None

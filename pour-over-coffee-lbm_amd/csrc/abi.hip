@@ -38,6 +38,11 @@ struct pol_ctx {
   double* stat_partial = nullptr;   // [nz * stat_tiles][POL_NSTAT]
   double* stat_zpart = nullptr;     // [nz * stat_tiles]
   double* stat_dev = nullptr;       // [POL_NSTAT + nz]
+  // thermal D3Q7
+  bool thermal = false;
+  pol_thermal_desc td{};
+  pol_thermal_buffers tb{};
+  int gcur = 0, tcur = 0;
   std::string err;
 };
 
@@ -577,6 +582,132 @@ int pol_stabilize(pol_ctx* c, double max_velocity, double min_density, double ma
     e = launch_stabilize(c->d.storage, a, (float)max_velocity, (float)min_density, (float)max_density, c->wgrid,
                          (hipStream_t)stream);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_stabilize");
+}
+
+// ---- thermal D3Q7 -----------------------------------------------------------
+static ThermalArgs thermal_args(const pol_ctx* c) {
+  ThermalArgs a{};
+  const pol_desc& d = c->d;
+  a.gin = c->tb.g[c->gcur];
+  a.gout = c->tb.g[c->gcur];
+  a.T = c->tb.temperature[c->tcur];
+  a.Tout = c->tb.temperature[c->tcur];
+  a.hs = c->tb.heat_source;
+  a.hsb = (c->td.options & POL_THERMAL_RESET_SOURCE) ? c->tb.heat_source_base : nullptr;
+  a.u = c->tb.velocity;
+  a.k = c->tb.conductivity;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.nz = d.nz;
+  a.nxy = (uint32_t)d.nx * d.ny;
+  a.N = c->N;
+  a.dnx = make_fastdiv(d.nx);
+  a.dny = make_fastdiv(d.ny);
+  a.omega_f = (float)c->td.omega;
+  a.omega_d = c->td.omega;
+  a.conv = (c->td.options & POL_THERMAL_CONVECTION) ? 1u : 0u;
+  a.partial = c->partial;
+  return a;
+}
+
+static int thermal_desc_ok(pol_ctx* c, const pol_thermal_desc* d, const char* where) {
+  if (!(d->omega > 0.0 && d->omega < 2.0))
+    return fail(c, POL_ERR_INVALID, std::string(where) + ": omega must lie in (0, 2)");
+  if (d->options & ~(POL_THERMAL_CONVECTION | POL_THERMAL_RESET_SOURCE))
+    return fail(c, POL_ERR_INVALID, std::string(where) + ": unknown options");
+  return POL_OK;
+}
+
+int pol_thermal_bind(pol_ctx* c, const pol_thermal_desc* d, const pol_thermal_buffers* b) {
+  if (!c || !d || !b) return POL_ERR_INVALID;
+  int st = thermal_desc_ok(c, d, "pol_thermal_bind");
+  if (st) return st;
+  if (!b->g[0] || !b->g[1] || b->g[0] == b->g[1] || !b->temperature[0] || !b->temperature[1] ||
+      b->temperature[0] == b->temperature[1] || !b->heat_source)
+    return fail(c, POL_ERR_INVALID, "pol_thermal_bind: two g buffers, two temperature buffers and heat_source required");
+  if ((d->options & POL_THERMAL_CONVECTION) && !b->velocity)
+    return fail(c, POL_ERR_INVALID, "pol_thermal_bind: convection needs the velocity buffer");
+  if ((d->options & POL_THERMAL_RESET_SOURCE) && !b->heat_source_base)
+    return fail(c, POL_ERR_INVALID, "pol_thermal_bind: POL_THERMAL_RESET_SOURCE needs heat_source_base");
+  c->td = *d;
+  c->tb = *b;
+  c->gcur = 0;
+  c->tcur = 0;
+  c->thermal = true;
+  return POL_OK;
+}
+
+int pol_thermal_set(pol_ctx* c, const pol_thermal_desc* d) {
+  if (!c || !d) return POL_ERR_INVALID;
+  if (!c->thermal) return fail(c, POL_ERR_STATE, "pol_thermal_set: thermal buffers not bound");
+  int st = thermal_desc_ok(c, d, "pol_thermal_set");
+  if (st) return st;
+  if ((d->options & POL_THERMAL_CONVECTION) && !c->tb.velocity)
+    return fail(c, POL_ERR_INVALID, "pol_thermal_set: convection needs the velocity buffer");
+  if ((d->options & POL_THERMAL_RESET_SOURCE) && !c->tb.heat_source_base)
+    return fail(c, POL_ERR_INVALID, "pol_thermal_set: POL_THERMAL_RESET_SOURCE needs heat_source_base");
+  c->td = *d;
+  return POL_OK;
+}
+
+int pol_thermal_init(pol_ctx* c, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->thermal) return fail(c, POL_ERR_STATE, "pol_thermal_init: thermal buffers not bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_thermal_init(c->d.storage == POL_F64, thermal_args(c), (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_thermal_init");
+}
+
+int pol_thermal_step(pol_ctx* c, int nsteps, int32_t* unstable_out, void* stream) {
+  if (!c || nsteps < 0) return POL_ERR_INVALID;
+  if (!c->thermal) return fail(c, POL_ERR_STATE, "pol_thermal_step: thermal buffers not bound");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_thermal_steps(c->d.storage == POL_F64, thermal_args(c), c->tb.g, c->tb.temperature, c->gcur, c->tcur,
+                             nsteps, s);
+  if (e == hipSuccess && unstable_out) {
+    if (nsteps == 0) {
+      *unstable_out = 0;
+    } else {
+      e = launch_thermal_fold(c->partial, (c->N + 255) / 256, 1, c->red_dev, s);
+      double v = 0.0;
+      if (e == hipSuccess) e = hipMemcpyAsync(&v, c->red_dev, sizeof(double), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      *unstable_out = (int32_t)v;
+    }
+  }
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_thermal_step");
+}
+
+int pol_thermal_heat_flux(pol_ctx* c, void* flux, void* stream) {
+  if (!c || !flux) return POL_ERR_INVALID;
+  if (!c->thermal) return fail(c, POL_ERR_STATE, "pol_thermal_heat_flux: thermal buffers not bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_thermal_flux(c->d.storage == POL_F64, thermal_args(c), flux, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_thermal_heat_flux");
+}
+
+int pol_thermal_stats(pol_ctx* c, double* out3, void* stream) {
+  if (!c || !out3) return POL_ERR_INVALID;
+  if (!c->thermal) return fail(c, POL_ERR_STATE, "pol_thermal_stats: thermal buffers not bound");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_thermal_stats(c->d.storage == POL_F64, c->tb.temperature[c->tcur], c->N, c->partial, s);
+  if (e == hipSuccess) e = launch_thermal_fold(c->partial, (c->N + 255) / 256, 3, c->red_dev, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(out3, c->red_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_thermal_stats");
+}
+
+int pol_thermal_boundary(pol_ctx* c, int kind, const uint8_t* mask, double a, double b, void* stream) {
+  if (!c || !mask || kind < 1 || kind > 3) return POL_ERR_INVALID;
+  if (!c->thermal) return fail(c, POL_ERR_STATE, "pol_thermal_boundary: thermal buffers not bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_thermal_boundary(c->d.storage == POL_F64, thermal_args(c), kind, mask, a, b, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_thermal_boundary");
 }
 
 }  // extern "C"

@@ -86,6 +86,34 @@ hipError_t launch_stabilize(int storage, const StepArgs& a, float max_u, float m
 hipError_t launch_export_fields(bool real_double, const void* rho, const void* u, uint32_t N, void* mag, void* pres,
                                 hipStream_t st);
 
+// Thermal D3Q7 state (thermal.hip), dense over every cell.
+struct ThermalArgs {
+  const void* gin;
+  void* gout;
+  const void* T;     // temperature of the state being stepped
+  void* Tout;        // temperature after the step
+  void* hs;          // heat source
+  const void* hsb;   // base heat source read instead of hs (reset fused), or null
+  const void* u;     // [3][N] or null
+  const void* k;     // conductivity or null
+  uint32_t nx, ny, nz, nxy, N;
+  FastDiv dnx, dny;
+  float omega_f;
+  double omega_d;
+  uint32_t conv;
+  double* partial;   // per-block partials (stability counts, statistics)
+};
+// nsteps steps ping-ponging g[gcur] / T[tcur]; base supplies the fields.
+hipError_t launch_thermal_steps(bool dbl, const ThermalArgs& base, void* const g[2], void* const T[2], int& gcur,
+                                int& tcur, int nsteps, hipStream_t s);
+// Fold per-block partials: nval 1 = sum into out[0]; 3 = {min, max, sum}.
+hipError_t launch_thermal_fold(const double* partial, uint32_t nblk, int nval, double* out, hipStream_t s);
+hipError_t launch_thermal_stats(bool dbl, const void* T, uint32_t N, double* partial, hipStream_t s);
+hipError_t launch_thermal_flux(bool dbl, const ThermalArgs& a, void* flux, hipStream_t s);
+hipError_t launch_thermal_init(bool dbl, const ThermalArgs& a, hipStream_t s);
+hipError_t launch_thermal_boundary(bool dbl, const ThermalArgs& a, int kind, const uint8_t* mask, double va,
+                                   double vb, hipStream_t s);
+
 // Particle state in the real type (float, or double with real_is_double).
 struct ParticleArgs {
   int32_t count;

@@ -115,13 +115,18 @@ int main(void) {
   F(pol_adjoint_buffers, rev_flags)
   S(pol_stats_opts) F(pol_stats_opts, prev_rho) F(pol_stats_opts, prev_u) F(pol_stats_opts, phase)
   F(pol_stats_opts, z_top) F(pol_stats_opts, z_mid) F(pol_stats_opts, z_outlet) F(pol_stats_opts, with_populations)
+  S(pol_thermal_desc) F(pol_thermal_desc, omega) F(pol_thermal_desc, options) F(pol_thermal_desc, reserved)
+  S(pol_thermal_buffers) F(pol_thermal_buffers, g) F(pol_thermal_buffers, temperature)
+  F(pol_thermal_buffers, heat_source) F(pol_thermal_buffers, velocity) F(pol_thermal_buffers, conductivity)
+  F(pol_thermal_buffers, heat_source_base)
   return 0;
 }
 """
 
 _CTYPES = {"pol_desc": L.PolDesc, "pol_buffers": L.PolBuffers, "pol_particle_desc": L.PolParticleDesc,
            "pol_particle_buffers": L.PolParticleBuffers, "pol_adjoint_buffers": L.PolAdjointBuffers,
-           "pol_stats_opts": L.PolStatsOpts}
+           "pol_stats_opts": L.PolStatsOpts, "pol_thermal_desc": L.PolThermalDesc,
+           "pol_thermal_buffers": L.PolThermalBuffers}
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
