@@ -12,11 +12,12 @@ no threads except the solid cells inside a row's run.
 Secondary lines in the same JSON object: the all-fluid periodic 224^3 box
 (every cell updated), FP16/BF16 storage speed-up over FP32, the two-way
 coupled step with 50 000 particles, adjoint cost per forward step, field
-diagnostics per call, the thermal D3Q7 step, and the CPU baseline (C
-restatements on the host cores, bounded samples).
+diagnostics per call, the thermal D3Q7 step, the phase-field passes, and the
+CPU baseline (C restatements on the host cores, bounded samples).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--precision P]
-       [--coupled-only | --adjoint-only | --diagnostics-only | --thermal-only] [--calibrate-copy]
+       [--coupled-only | --adjoint-only | --diagnostics-only | --thermal-only | --multiphase-only]
+       [--calibrate-copy]
 """
 from __future__ import annotations
 
@@ -400,6 +401,112 @@ def bench_thermal(n=224, steps=200, warmup=20, cpu=True):
     return out
 
 
+def bench_multiphase(n=224, reps=100, warmup=10, cpu=True):
+    """Phase field on the 224^3 V60 domain, FP32 (multiphase_3d.py:389-421
+    in the main loop's order, main.py:769-839): the pre-collision pass
+    (gradients, curvature, CSF force, body_force += F / rho) and the phase
+    step (advection, M lap(mu), separation, copy, density / phase), timed
+    with HIP events on the stream they run on; then the whole main-loop
+    step (clear body force, pre-collision pass, stream-collide writing rho /
+    u, phase step).  Initial state: water below z = n/4 and a droplet of
+    radius n/10, after 20 fluid steps.  CPU baseline: the numpy
+    restatement (oracle/multiphase.py, one core) on a 96^3 sample."""
+    import numpy as np
+    import torch
+
+    from pourover_amd import _lib as L
+    from pourover_amd import multiphase as MP
+    from pourover_amd import workloads as WL
+    from pourover_amd.engine import HipContext
+
+    wl = WL.v60_config(n, "f32", force_field=True)
+    ctx = HipContext(wl.config, wl.geometry.flags)
+    ctx.init_equilibrium()
+    ctx.step(20)
+    ctx.macroscopic(macro=True)
+    mp = MP.MultiphaseFlow3D(ctx)
+    z = torch.arange(n, device="cuda", dtype=torch.float32).view(-1, 1, 1)
+    y = torch.arange(n, device="cuda", dtype=torch.float32).view(1, -1, 1)
+    x = torch.arange(n, device="cuda", dtype=torch.float32).view(1, 1, -1)
+    r = torch.sqrt((x - n / 2) ** 2 + (y - n / 2) ** 2 + (z - 0.6 * n) ** 2)
+    phi = torch.maximum(torch.tanh((n / 4 - z) / 2), torch.tanh((n / 10 - r) / 2))
+    mp._phi[0].copy_(phi.expand(n, n, n))
+    mp._phi_new.fill_(-1.0)
+    mp.standardize_initial_state(force_dry_state=False)
+    torch.cuda.synchronize()
+
+    def timed(fn, k):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(k):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / 1e3 / k
+
+    def phase_step():
+        L.check(mp.lib.pol_multiphase_step(ctx.handle, L.stream_handle()), ctx.handle, "pol_multiphase_step")
+        mp._cur ^= 1
+
+    def main_loop_step():
+        ctx.force.zero_()
+        mp.accumulate_surface_tension_pre_collision()
+        ctx.step(1, L.POL_WANT_MACRO)
+        phase_step()
+
+    t_if = timed(mp.accumulate_surface_tension_pre_collision, reps)
+    t_ph = timed(phase_step, reps)
+    t_loop = timed(main_loop_step, reps)
+    t_fluid = timed(lambda: ctx.step(1, L.POL_WANT_MACRO), reps)
+    N = n ** 3
+    solid = torch.from_numpy(wl.geometry.kind == L.POL_KIND_SOLID).to("cuda")
+    nsolid = int(solid.sum().item())
+    n_if = int(((mp._phi[mp._cur].abs() < 0.9) & ~solid).sum().item())
+    # flags everywhere; phi + body force read on fluid cells; rho read and the
+    # body force written where a force can arise (fluid cells with |phi| < 0.9)
+    b_if = 4 * N + 16 * (N - nsolid) + 16 * n_if
+    b_ph = 32 * N                       # phi, mu, u read; phi, rho, phase written
+    out = {"grid": [n, n, n], "cells": N, "solid_cells": nsolid, "interface_fluid_cells": n_if,
+           "pre_collision_pass": {"ms": round(t_if * 1e3, 4), "algorithmic_bytes": b_if,
+                                  "achieved_GBs": round(b_if / t_if / 1e9, 1),
+                                  "frac": round(b_if / t_if / 1e9 / HBM_PEAK_GBS, 4)},
+           "phase_step": {"ms": round(t_ph * 1e3, 4), "algorithmic_bytes": b_ph,
+                          "achieved_GBs": round(b_ph / t_ph / 1e9, 1),
+                          "frac": round(b_ph / t_ph / 1e9 / HBM_PEAK_GBS, 4)},
+           "main_loop_step_ms": round(t_loop * 1e3, 4),
+           "fluid_step_with_macro_ms": round(t_fluid * 1e3, 4),
+           "interface": mp.get_interface_statistics()}
+    del mp, ctx
+    torch.cuda.empty_cache()
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import multiphase as OM
+
+        m = 96
+        g = np.meshgrid(*(np.arange(m, dtype=np.float32),) * 3, indexing="ij")
+        rr = np.sqrt((g[0] - m / 2) ** 2 + (g[1] - m / 2) ** 2 + (g[2] - 0.6 * m) ** 2)
+        ph = np.maximum(np.tanh((m / 4 - g[2]) / 2), np.tanh((m / 10 - rr) / 2)).astype(np.float32)
+        rng = np.random.default_rng(2)
+        s = OM.new_state(ph, (rng.random((m, m, m, 3), dtype=np.float32) - 0.5) * 0.02,
+                         np.ones((m, m, m), np.float32), np.zeros((m, m, m), np.uint8))
+        ns, t0 = 0, time.perf_counter()
+        while True:
+            OM.accumulate_surface_tension_pre_collision(s)
+            OM.step(s, 20, precollision_applied=True)
+            ns += 1
+            el = time.perf_counter() - t0
+            if el > 8.0 or ns >= 30:
+                break
+        out["cpu_baseline"] = {"value": round(m ** 3 * ns / el / 1e6, 2), "unit": "M cell updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "%d pre-collision passes + steps of a %d^3 box, numpy restatement of "
+                                         "multiphase_3d.py, one core" % (ns, m)}
+    return out
+
+
 def counter_traffic(precision):
     """HBM bytes per launch of the stream-collide kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
@@ -428,6 +535,8 @@ def main():
                     help="only the 224^3 field-diagnostics measurement (for profiling)")
     ap.add_argument("--thermal-only", action="store_true",
                     help="only the 224^3 thermal D3Q7 measurement (for profiling)")
+    ap.add_argument("--multiphase-only", action="store_true",
+                    help="only the 224^3 phase-field measurement (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
@@ -457,6 +566,11 @@ def main():
     if args.diagnostics_only:
         log("bench: field diagnostics 224^3 f32")
         print(json.dumps({"diagnostics_224_f32": bench_diagnostics(host=False)}), flush=True)
+        return
+    if args.multiphase_only:
+        log("bench: multiphase 224^3 f32")
+        print(json.dumps({"multiphase_224_f32": bench_multiphase(reps=args.steps, warmup=args.warmup, cpu=False)}),
+              flush=True)
         return
     if args.thermal_only:
         log("bench: thermal D3Q7 224^3 f32")
@@ -518,6 +632,8 @@ def main():
         out["diagnostics_224_f32"] = bench_diagnostics()
         log("bench: thermal D3Q7 224^3 f32")
         out["thermal_224_f32"] = bench_thermal(steps=args.steps, warmup=args.warmup)
+        log("bench: multiphase 224^3 f32")
+        out["multiphase_224_f32"] = bench_multiphase(reps=min(args.steps, 100), warmup=args.warmup)
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -440,6 +440,63 @@ int pol_thermal_stats(pol_ctx *ctx, double *out3, void *stream);
  * (S = a), kind 3 convective (S = -a (T - b)). */
 int pol_thermal_boundary(pol_ctx *ctx, int kind, const uint8_t *mask, double a, double b, void *stream);
 
+/* ---- multiphase phase field (src/core/multiphase_3d.py) ---------------------
+ * Cahn-Hilliard phase field phi (-1 air, +1 water) with CSF surface tension,
+ * the producer of the rho / phase / body_force the stream-collide step reads
+ * (main.py:795-800, 837-839).  Fields are float32 [cells] like the
+ * reference's; rho, u and the body force are the bound context's buffers
+ * (u as the last pol_macroscopic left it), phase is real [cells].  The
+ * reference's step (multiphase_3d.py:389-414) never writes face cells:
+ * their normal and surface force stay 0 and phi takes their phi_new values.
+ * float32 arithmetic in the reference's operation order, no contraction. */
+#define POL_MP_APPLY 0x1u /* pol_multiphase_interface: body_force += F / rho on fluid cells with rho > 1e-10 */
+
+typedef struct pol_multiphase_desc {
+  double sigma;           /* SURFACE_TENSION_COEFF (config.SURFACE_TENSION_LU) */
+  double mobility;        /* MOBILITY M of M lap(mu) */
+  double interface_width; /* INTERFACE_WIDTH W: kappa = 3 sigma W / 8 in the chemical potential */
+  double rho_water;       /* config.RHO_WATER */
+  double rho_air;         /* config.RHO_AIR */
+  double dt;              /* config.DT */
+} pol_multiphase_desc;
+
+typedef struct pol_multiphase_buffers {
+  float *phi[2];          /* [cells] phase field, ping-pong (phi[0] current after bind) */
+  const float *phi_new;   /* [cells] the reference's phi_new: the step copies its face cells into phi */
+  float *mu;              /* [cells] chemical potential */
+  void *phase;            /* real [cells] lbm.phase, written by the step */
+} pol_multiphase_buffers;
+
+/* Optional outputs of pol_multiphase_interface (each may be NULL): the
+ * reference's grad_phi, grad_mu, normal ([3][cells]), curvature ([cells])
+ * and surface_force ([3][cells]) on interior cells. */
+typedef struct pol_interface_fields {
+  float *grad_phi;
+  float *grad_mu;
+  float *normal;
+  float *curvature;
+  float *surface_force;
+} pol_interface_fields;
+
+/* Attach the phase-field buffers to a bound context. */
+int pol_multiphase_bind(pol_ctx *ctx, const pol_multiphase_desc *d, const pol_multiphase_buffers *b);
+/* compute_chemical_potential (multiphase_3d.py:78-108): mu (and
+ * laplacian_phi when not NULL) on interior cells of the current phi. */
+int pol_multiphase_chemical_potential(pol_ctx *ctx, float *laplacian_phi, void *stream);
+/* compute_gradients + compute_curvature + compute_surface_tension_force
+ * (multiphase_3d.py:110-149, 365-381) of the current phi; with POL_MP_APPLY
+ * also apply_surface_tension (:403-411) into the context's force buffer --
+ * accumulate_surface_tension_pre_collision (:416-421) in one pass. */
+int pol_multiphase_interface(pol_ctx *ctx, uint32_t flags, const pol_interface_fields *out, void *stream);
+/* The rest of step (multiphase_3d.py:389-414): upwind advection with the
+ * context's u, M lap(mu), clamp, phase separation, copy to phi (the
+ * ping-pong flips) and update_density_from_phase into rho and phase. */
+int pol_multiphase_step(pol_ctx *ctx, void *stream);
+/* update_density_from_phase alone (multiphase_3d.py:413-428). */
+int pol_multiphase_density(pol_ctx *ctx, void *stream);
+/* Index (0 / 1) of the current phi buffer. */
+int pol_multiphase_current(pol_ctx *ctx, int *index);
+
 #ifdef __cplusplus
 }
 #endif

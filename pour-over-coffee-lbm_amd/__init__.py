@@ -1,7 +1,7 @@
 """MI355X-native D3Q19 lattice-Boltzmann engine (pour-over coffee brewer).
 
 Hot path: hand-written gfx950 HIP kernels in csrc/ (fused stream-collide,
-particle coupling, adjoint, field diagnostics, thermal D3Q7), driven
+particle coupling, adjoint, field diagnostics, thermal D3Q7, phase field), driven
 through the C ABI of include/pourover_hip.h via ctypes.  Host layer mirrors the reference's
 backend plugin API (backends.py) and solver protocol (solver.py).
 
@@ -11,7 +11,7 @@ first use and raises ImportError if it has not been built (no CPU fallback).
 from . import lattice  # noqa: F401
 
 __all__ = ["lattice", "geometry", "engine", "solver", "backends", "fields", "workloads", "particles", "adjoint",
-           "diagnostics", "thermal"]
+           "diagnostics", "thermal", "multiphase"]
 __version__ = "0.1.0"
 
 

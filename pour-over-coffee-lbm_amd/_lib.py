@@ -96,6 +96,7 @@ STAT_SLOTS = ("cells", "sum_drho", "sum_drho2", "max_abs_drho", "mom_x", "mom_y"
 
 POL_THERMAL_CONVECTION = 0x1
 POL_THERMAL_RESET_SOURCE = 0x2
+POL_MP_APPLY = 0x1
 
 STORAGE_CODES = {"f32": POL_F32, "f16": POL_F16, "bf16": POL_BF16, "f64": POL_F64}
 
@@ -149,6 +150,20 @@ class PolThermalBuffers(C.Structure):
                 ("velocity", C.c_void_p), ("conductivity", C.c_void_p), ("heat_source_base", C.c_void_p)]
 
 
+class PolMultiphaseDesc(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("mobility", C.c_double), ("interface_width", C.c_double),
+                ("rho_water", C.c_double), ("rho_air", C.c_double), ("dt", C.c_double)]
+
+
+class PolMultiphaseBuffers(C.Structure):
+    _fields_ = [("phi", C.c_void_p * 2), ("phi_new", C.c_void_p), ("mu", C.c_void_p), ("phase", C.c_void_p)]
+
+
+class PolInterfaceFields(C.Structure):
+    _fields_ = [("grad_phi", C.c_void_p), ("grad_mu", C.c_void_p), ("normal", C.c_void_p), ("curvature", C.c_void_p),
+                ("surface_force", C.c_void_p)]
+
+
 class PolAdjointBuffers(C.Structure):
     _fields_ = [("lam", C.c_void_p * 2), ("grad_perm", C.c_void_p), ("rev_flags", C.c_void_p)]
 
@@ -191,6 +206,12 @@ SIGNATURES = {
     "pol_thermal_heat_flux": (C.c_int, [_P, _P, _P]),
     "pol_thermal_stats": (C.c_int, [_P, _P, _P]),
     "pol_thermal_boundary": (C.c_int, [_P, C.c_int, _P, C.c_double, C.c_double, _P]),
+    "pol_multiphase_bind": (C.c_int, [_P, C.POINTER(PolMultiphaseDesc), C.POINTER(PolMultiphaseBuffers)]),
+    "pol_multiphase_chemical_potential": (C.c_int, [_P, _P, _P]),
+    "pol_multiphase_interface": (C.c_int, [_P, C.c_uint32, C.POINTER(PolInterfaceFields), _P]),
+    "pol_multiphase_step": (C.c_int, [_P, _P]),
+    "pol_multiphase_density": (C.c_int, [_P, _P]),
+    "pol_multiphase_current": (C.c_int, [_P, C.POINTER(C.c_int)]),
 }
 
 _lib = None

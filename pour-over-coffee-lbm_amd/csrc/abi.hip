@@ -38,6 +38,11 @@ struct pol_ctx {
   double* stat_partial = nullptr;   // [nz * stat_tiles][POL_NSTAT]
   double* stat_zpart = nullptr;     // [nz * stat_tiles]
   double* stat_dev = nullptr;       // [POL_NSTAT + nz]
+  // multiphase phase field
+  bool mp = false;
+  pol_multiphase_desc md{};
+  pol_multiphase_buffers mb{};
+  int mcur = 0;
   // thermal D3Q7
   bool thermal = false;
   pol_thermal_desc td{};
@@ -582,6 +587,107 @@ int pol_stabilize(pol_ctx* c, double max_velocity, double min_density, double ma
     e = launch_stabilize(c->d.storage, a, (float)max_velocity, (float)min_density, (float)max_density, c->wgrid,
                          (hipStream_t)stream);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_stabilize");
+}
+
+// ---- multiphase phase field -----------------------------------------------------
+static MultiphaseArgs mp_args(const pol_ctx* c) {
+  MultiphaseArgs a{};
+  const pol_desc& d = c->d;
+  a.phi = c->mb.phi[c->mcur];
+  a.phi_out = c->mb.phi[1 - c->mcur];
+  a.phi_new = c->mb.phi_new;
+  a.mu = c->mb.mu;
+  a.u = c->b.u;
+  a.rho = c->b.rho;
+  a.phase = c->mb.phase;
+  a.force = (d.options & POL_FORCE_FIELD) ? c->b.force : nullptr;
+  a.flags = (d.options & POL_USE_FLAGS) ? c->b.cell_flags : nullptr;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.nz = d.nz;
+  a.nxy = (uint32_t)d.nx * d.ny;
+  a.N = c->N;
+  a.dnx = make_fastdiv(d.nx);
+  a.dny = make_fastdiv(d.ny);
+  const pol_multiphase_desc& m = c->md;
+  a.sigma = (float)m.sigma;
+  a.mobility = (float)m.mobility;
+  a.dt = (float)m.dt;
+  a.kappa = (float)(3.0 * m.sigma * m.interface_width / 8.0);
+  a.rho_air = (float)m.rho_air;
+  a.drho = (float)(m.rho_water - m.rho_air);
+  return a;
+}
+
+int pol_multiphase_bind(pol_ctx* c, const pol_multiphase_desc* d, const pol_multiphase_buffers* b) {
+  if (!c || !d || !b) return POL_ERR_INVALID;
+  if (!c->bound || !c->b.rho || !c->b.u)
+    return fail(c, POL_ERR_STATE, "pol_multiphase_bind: bind the context (rho and u buffers) first");
+  if (!b->phi[0] || !b->phi[1] || b->phi[0] == b->phi[1] || !b->phi_new || !b->mu || !b->phase)
+    return fail(c, POL_ERR_INVALID, "pol_multiphase_bind: two phi buffers, phi_new, mu and phase required");
+  if (!std::isfinite(d->sigma) || !std::isfinite(d->mobility) || !std::isfinite(d->interface_width) ||
+      !std::isfinite(d->rho_water) || !std::isfinite(d->rho_air) || !std::isfinite(d->dt))
+    return fail(c, POL_ERR_INVALID, "pol_multiphase_bind: non-finite parameter");
+  c->md = *d;
+  c->mb = *b;
+  c->mcur = 0;
+  c->mp = true;
+  return POL_OK;
+}
+
+int pol_multiphase_chemical_potential(pol_ctx* c, float* laplacian_phi, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->mp) return fail(c, POL_ERR_STATE, "pol_multiphase_chemical_potential: phase field not bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_mp_chemical_potential(mp_args(c), laplacian_phi, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_multiphase_chemical_potential");
+}
+
+int pol_multiphase_interface(pol_ctx* c, uint32_t flags, const pol_interface_fields* out, void* stream) {
+  if (!c || (flags & ~POL_MP_APPLY)) return POL_ERR_INVALID;
+  if (!c->mp) return fail(c, POL_ERR_STATE, "pol_multiphase_interface: phase field not bound");
+  MultiphaseArgs a = mp_args(c);
+  a.apply = (flags & POL_MP_APPLY) ? 1u : 0u;
+  if (a.apply && !a.force)
+    return fail(c, POL_ERR_STATE, "pol_multiphase_interface: POL_MP_APPLY needs the context's force field");
+  bool any = false;
+  if (out) {
+    a.grad_phi = out->grad_phi;
+    a.grad_mu = out->grad_mu;
+    a.normal = out->normal;
+    a.curvature = out->curvature;
+    a.sforce = out->surface_force;
+    any = a.grad_phi || a.grad_mu || a.normal || a.curvature || a.sforce;
+  }
+  if (!any && !a.apply) return POL_OK;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_mp_interface(c->d.storage == POL_F64, a, any, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_multiphase_interface");
+}
+
+int pol_multiphase_step(pol_ctx* c, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->mp) return fail(c, POL_ERR_STATE, "pol_multiphase_step: phase field not bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_mp_phase(c->d.storage == POL_F64, mp_args(c), (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "pol_multiphase_step");
+  c->mcur = 1 - c->mcur;
+  return POL_OK;
+}
+
+int pol_multiphase_density(pol_ctx* c, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->mp) return fail(c, POL_ERR_STATE, "pol_multiphase_density: phase field not bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_mp_density(c->d.storage == POL_F64, mp_args(c), (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_multiphase_density");
+}
+
+int pol_multiphase_current(pol_ctx* c, int* index) {
+  if (!c || !index) return POL_ERR_INVALID;
+  if (!c->mp) return fail(c, POL_ERR_STATE, "pol_multiphase_current: phase field not bound");
+  *index = c->mcur;
+  return POL_OK;
 }
 
 // ---- thermal D3Q7 -----------------------------------------------------------

@@ -86,6 +86,32 @@ hipError_t launch_stabilize(int storage, const StepArgs& a, float max_u, float m
 hipError_t launch_export_fields(bool real_double, const void* rho, const void* u, uint32_t N, void* mag, void* pres,
                                 hipStream_t st);
 
+// Phase-field producer (multiphase.hip), dense over every cell.
+struct MultiphaseArgs {
+  const float* phi;      // current phase field
+  float* phi_out;        // next phase field (step)
+  const float* phi_new;  // face values the step copies
+  float* mu;
+  const void* u;         // real [3][N]
+  void* rho;             // real [N]
+  void* phase;           // real [N]
+  void* force;           // real [3][N] body force (apply)
+  const uint32_t* flags; // cell flags or null (no solids)
+  float* grad_phi;       // optional outputs (interface kernel)
+  float* grad_mu;
+  float* normal;
+  float* curvature;
+  float* sforce;
+  uint32_t nx, ny, nz, nxy, N;
+  FastDiv dnx, dny;
+  float sigma, mobility, dt, kappa, rho_air, drho;
+  uint32_t apply;
+};
+hipError_t launch_mp_interface(bool dbl, const MultiphaseArgs& a, bool out, hipStream_t s);
+hipError_t launch_mp_phase(bool dbl, const MultiphaseArgs& a, hipStream_t s);
+hipError_t launch_mp_density(bool dbl, const MultiphaseArgs& a, hipStream_t s);
+hipError_t launch_mp_chemical_potential(const MultiphaseArgs& a, float* lap_out, hipStream_t s);
+
 // Thermal D3Q7 state (thermal.hip), dense over every cell.
 struct ThermalArgs {
   const void* gin;

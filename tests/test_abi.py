@@ -119,6 +119,13 @@ int main(void) {
   S(pol_thermal_buffers) F(pol_thermal_buffers, g) F(pol_thermal_buffers, temperature)
   F(pol_thermal_buffers, heat_source) F(pol_thermal_buffers, velocity) F(pol_thermal_buffers, conductivity)
   F(pol_thermal_buffers, heat_source_base)
+  S(pol_multiphase_desc) F(pol_multiphase_desc, sigma) F(pol_multiphase_desc, mobility)
+  F(pol_multiphase_desc, interface_width) F(pol_multiphase_desc, rho_water) F(pol_multiphase_desc, rho_air)
+  F(pol_multiphase_desc, dt)
+  S(pol_multiphase_buffers) F(pol_multiphase_buffers, phi) F(pol_multiphase_buffers, phi_new)
+  F(pol_multiphase_buffers, mu) F(pol_multiphase_buffers, phase)
+  S(pol_interface_fields) F(pol_interface_fields, grad_phi) F(pol_interface_fields, grad_mu)
+  F(pol_interface_fields, normal) F(pol_interface_fields, curvature) F(pol_interface_fields, surface_force)
   return 0;
 }
 """
@@ -126,7 +133,8 @@ int main(void) {
 _CTYPES = {"pol_desc": L.PolDesc, "pol_buffers": L.PolBuffers, "pol_particle_desc": L.PolParticleDesc,
            "pol_particle_buffers": L.PolParticleBuffers, "pol_adjoint_buffers": L.PolAdjointBuffers,
            "pol_stats_opts": L.PolStatsOpts, "pol_thermal_desc": L.PolThermalDesc,
-           "pol_thermal_buffers": L.PolThermalBuffers}
+           "pol_thermal_buffers": L.PolThermalBuffers, "pol_multiphase_desc": L.PolMultiphaseDesc,
+           "pol_multiphase_buffers": L.PolMultiphaseBuffers, "pol_interface_fields": L.PolInterfaceFields}
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
