@@ -12,11 +12,13 @@ no threads except the solid cells inside a row's run.
 Secondary lines in the same JSON object: the all-fluid periodic 224^3 box
 (every cell updated), FP16/BF16 storage speed-up over FP32, the two-way
 coupled step with 50 000 particles, adjoint cost per forward step, field
-diagnostics per call, the thermal D3Q7 step, the phase-field passes, and the
-CPU baseline (C restatements on the host cores, bounded samples).
+diagnostics per call, the thermal D3Q7 step, the phase-field passes, the
+body-force producers, and the CPU baseline (C restatements on the host
+cores, bounded samples).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--precision P]
-       [--coupled-only | --adjoint-only | --diagnostics-only | --thermal-only | --multiphase-only]
+       [--coupled-only | --adjoint-only | --diagnostics-only | --thermal-only | --multiphase-only |
+        --forcing-only]
        [--calibrate-copy]
 """
 from __future__ import annotations
@@ -507,6 +509,72 @@ def bench_multiphase(n=224, reps=100, warmup=10, cpu=True):
     return out
 
 
+def bench_forcing(n=224, reps=100, warmup=10):
+    """Body-force producers on the 224^3 V60 domain, FP32, one call each:
+    the pour force and the pour phase change (nozzle at the cup axis,
+    bounding box of the pour region only), the pressure drive in force mode
+    (gradient, limit and accumulation in one pass), the density drive, and
+    the filter blockage update over the paper cells.  HIP events on the
+    stream they run on."""
+    import numpy as np
+    import torch
+
+    from pourover_amd import _lib as L
+    from pourover_amd import forcing as FC
+    from pourover_amd import workloads as WL
+    from pourover_amd.engine import HipContext
+
+    wl = WL.v60_config(n, "f32", force_field=True)
+    ctx = HipContext(wl.config, wl.geometry.flags)
+    ctx.init_equilibrium()
+    ctx.step(20)
+    ctx.macroscopic(macro=True)
+    pour = FC.PrecisePouringSystem(ctx)
+    pour.start_pouring()
+    phi = torch.full((n, n, n), -1.0, device="cuda")
+    drive = FC.PressureGradientDrive(ctx)
+    drive.activate_force_drive(True)
+    filt = FC.FilterPaperSystem(ctx, wl.geometry.porous)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / 1e3 / reps
+
+    def density():
+        L.check(ctx.lib.pol_density_drive(ctx.handle, drive._target.data_ptr(), 0.025, 0, 0.001, 0.5, 2.0,
+                                          L.stream_handle()), ctx.handle, "pol_density_drive")
+
+    N = n ** 3
+    nsolid = int((wl.geometry.kind == L.POL_KIND_SOLID).sum())
+    nf = N - nsolid
+    nzone = int(np.count_nonzero(wl.geometry.porous))
+    out = {"grid": [n, n, n], "fluid_cells": nf, "filter_cells": nzone}
+    for name, fn, algo in (
+            ("pour_force", lambda: pour.apply_pouring_force(1.0), None),
+            ("pour_phase", lambda: pour.apply_gradual_phase_change(phi, None, 1.0), None),
+            ("pressure_force_drive", drive.apply_force_drive, 4 * N + 40 * nf),
+            ("density_drive", density, 4 * N + 12 * nf),
+            ("filter_blockage", filt.update_dynamic_resistance, N + 16 * nzone)):
+        t = timed(fn)
+        rec = {"ms": round(t * 1e3, 4)}
+        if algo:
+            rec.update({"algorithmic_bytes": algo, "achieved_GBs": round(algo / t / 1e9, 1),
+                        "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4)})
+        out[name] = rec
+    del ctx
+    torch.cuda.empty_cache()
+    return out
+
+
 def counter_traffic(precision):
     """HBM bytes per launch of the stream-collide kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
@@ -537,6 +605,8 @@ def main():
                     help="only the 224^3 thermal D3Q7 measurement (for profiling)")
     ap.add_argument("--multiphase-only", action="store_true",
                     help="only the 224^3 phase-field measurement (for profiling)")
+    ap.add_argument("--forcing-only", action="store_true",
+                    help="only the 224^3 body-force producer measurement (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
@@ -571,6 +641,10 @@ def main():
         log("bench: multiphase 224^3 f32")
         print(json.dumps({"multiphase_224_f32": bench_multiphase(reps=args.steps, warmup=args.warmup, cpu=False)}),
               flush=True)
+        return
+    if args.forcing_only:
+        log("bench: body-force producers 224^3 f32")
+        print(json.dumps({"forcing_224_f32": bench_forcing(reps=args.steps, warmup=args.warmup)}), flush=True)
         return
     if args.thermal_only:
         log("bench: thermal D3Q7 224^3 f32")
@@ -634,6 +708,8 @@ def main():
         out["thermal_224_f32"] = bench_thermal(steps=args.steps, warmup=args.warmup)
         log("bench: multiphase 224^3 f32")
         out["multiphase_224_f32"] = bench_multiphase(reps=min(args.steps, 100), warmup=args.warmup)
+        log("bench: body-force producers 224^3 f32")
+        out["forcing_224_f32"] = bench_forcing(reps=min(args.steps, 100), warmup=args.warmup)
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

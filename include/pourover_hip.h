@@ -497,6 +497,50 @@ int pol_multiphase_density(pol_ctx *ctx, void *stream);
 /* Index (0 / 1) of the current phi buffer. */
 int pol_multiphase_current(pol_ctx *ctx, int *index);
 
+/* ---- body-force producers of the main loop ------------------------------------
+ * Pouring (src/physics/precise_pouring.py), pressure drive
+ * (src/physics/pressure_gradient_drive.py) and filter-paper dynamics
+ * (src/physics/filter_paper.py) on the bound context's rho / body force /
+ * cell flags (solid cells are skipped, as the reference skips solid == 1).
+ * float32 arithmetic in the reference's operation order, no contraction. */
+typedef struct pol_pour_desc {
+  double center_x, center_y; /* pour position this step (lattice units) */
+  double radius;             /* POUR_DIAMETER_GRID / 2 */
+  double stream_height;      /* pour_stream_height (4.0) */
+  double velocity;           /* POUR_VELOCITY (lu / ts) */
+  double flow_rate;          /* pour_flow_rate */
+  double dt;                 /* time step of this call */
+  int32_t height;            /* POUR_HEIGHT: z of the nozzle */
+  int32_t reserved;
+} pol_pour_desc;
+/* apply_pouring_force (precise_pouring.py:132-162): body_force.z -=
+ * min(velocity I flow_rate / dt, 10) on fluid cells of the pour region,
+ * I = exp(-0.5 (d / radius)^2) exp(-(height - z) / 2).  Only the region's
+ * bounding box is launched. */
+int pol_pour_force(pol_ctx *ctx, const pol_pour_desc *d, void *stream);
+/* apply_gradual_phase_change (precise_pouring.py:164-193) on phi (float32
+ * [cells]). */
+int pol_pour_phase(pol_ctx *ctx, const pol_pour_desc *d, float *phi, void *stream);
+/* apply_density_drive (pressure_gradient_drive.py:96-122; half != 0: the
+ * mixed form :206-227): rho += limited(rate (target - rho)) clamped to
+ * [lo, hi] on fluid cells; target is float32 [cells]. */
+int pol_density_drive(pol_ctx *ctx, const float *target, double rate, int half, double max_adjust, double lo,
+                      double hi, void *stream);
+/* compute_pressure_gradient (pressure_gradient_drive.py:124-173) into
+ * pressure_force (float32 [3][cells], fluid cells) and, mode 1, body_force +=
+ * pressure_force (:186-190) or, mode 2, body_force += 0.5 pressure_force
+ * (:276-279); mode 0 stores only. */
+int pol_pressure_force(pol_ctx *ctx, float *pressure_force, int mode, double max_force, double cs2, void *stream);
+/* update_dynamic_resistance (filter_paper.py:699-746) on cells with
+ * zone == 1 (uint8 [cells]); accumulated / blockage float32 [cells]. */
+int pol_filter_blockage(pol_ctx *ctx, const uint8_t *zone, float *accumulated, float *blockage, void *stream);
+/* block_particles_at_filter (filter_paper.py:616-697): float32 SoA
+ * positions / velocities [3][count], int32 active; grid cell = int(pos /
+ * scale).  The horizontal kick draws from a counter-based generator keyed
+ * by seed and particle index (the reference uses ti.random). */
+int pol_filter_block_particles(pol_ctx *ctx, const uint8_t *zone, float *accumulated, const float *pos, float *vel,
+                               const int32_t *active, int32_t count, double scale, uint32_t seed, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

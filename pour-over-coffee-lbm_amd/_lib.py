@@ -164,6 +164,12 @@ class PolInterfaceFields(C.Structure):
                 ("surface_force", C.c_void_p)]
 
 
+class PolPourDesc(C.Structure):
+    _fields_ = [("center_x", C.c_double), ("center_y", C.c_double), ("radius", C.c_double),
+                ("stream_height", C.c_double), ("velocity", C.c_double), ("flow_rate", C.c_double),
+                ("dt", C.c_double), ("height", C.c_int32), ("reserved", C.c_int32)]
+
+
 class PolAdjointBuffers(C.Structure):
     _fields_ = [("lam", C.c_void_p * 2), ("grad_perm", C.c_void_p), ("rev_flags", C.c_void_p)]
 
@@ -212,6 +218,12 @@ SIGNATURES = {
     "pol_multiphase_step": (C.c_int, [_P, _P]),
     "pol_multiphase_density": (C.c_int, [_P, _P]),
     "pol_multiphase_current": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "pol_pour_force": (C.c_int, [_P, C.POINTER(PolPourDesc), _P]),
+    "pol_pour_phase": (C.c_int, [_P, C.POINTER(PolPourDesc), _P, _P]),
+    "pol_density_drive": (C.c_int, [_P, _P, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, _P]),
+    "pol_pressure_force": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, _P]),
+    "pol_filter_blockage": (C.c_int, [_P, _P, _P, _P, _P]),
+    "pol_filter_block_particles": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_uint32, _P]),
 }
 
 _lib = None

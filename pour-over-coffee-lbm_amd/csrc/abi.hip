@@ -690,6 +690,135 @@ int pol_multiphase_current(pol_ctx* c, int* index) {
   return POL_OK;
 }
 
+// ---- body-force producers ---------------------------------------------------------
+static ForcingArgs forcing_args(const pol_ctx* c) {
+  ForcingArgs f{};
+  const pol_desc& d = c->d;
+  f.force = (d.options & POL_FORCE_FIELD) ? c->b.force : nullptr;
+  f.rho = c->b.rho;
+  f.flags = (d.options & POL_USE_FLAGS) ? c->b.cell_flags : nullptr;
+  f.nx = d.nx;
+  f.ny = d.ny;
+  f.nz = d.nz;
+  f.nxy = (uint32_t)d.nx * d.ny;
+  f.N = c->N;
+  f.dnx = make_fastdiv(d.nx);
+  f.dny = make_fastdiv(d.ny);
+  return f;
+}
+
+static int pour_args(pol_ctx* c, const pol_pour_desc* d, PourArgs& p, const char* where) {
+  if (!std::isfinite(d->center_x) || !std::isfinite(d->center_y) || !(d->radius >= 0.0) ||
+      !std::isfinite(d->radius) || !std::isfinite(d->stream_height) || !std::isfinite(d->velocity) ||
+      !std::isfinite(d->flow_rate) || !std::isfinite(d->dt))
+    return fail(c, POL_ERR_INVALID, std::string(where) + ": non-finite pour parameter");
+  p = PourArgs{};
+  p.cx = (float)d->center_x;
+  p.cy = (float)d->center_y;
+  p.radius = (float)d->radius;
+  p.zmin = (float)((double)d->height - d->stream_height);
+  p.velocity = (float)d->velocity;
+  p.flow_rate = (float)d->flow_rate;
+  p.dt = (float)d->dt;
+  p.height = d->height;
+  // bounding box of |(i, j) - c| <= radius, zmin <= k <= height, one cell of margin
+  auto lo = [](double v) { return (int64_t)std::floor(v) - 1; };
+  auto hi = [](double v) { return (int64_t)std::ceil(v) + 1; };
+  const int64_t x0 = std::max<int64_t>(0, lo((double)p.cx - p.radius));
+  const int64_t x1 = std::min<int64_t>(c->d.nx - 1, hi((double)p.cx + p.radius));
+  const int64_t y0 = std::max<int64_t>(0, lo((double)p.cy - p.radius));
+  const int64_t y1 = std::min<int64_t>(c->d.ny - 1, hi((double)p.cy + p.radius));
+  const int64_t z0 = std::max<int64_t>(0, lo((double)p.zmin));
+  const int64_t z1 = std::min<int64_t>(c->d.nz - 1, (int64_t)d->height);
+  p.x0 = (int32_t)x0;
+  p.y0 = (int32_t)y0;
+  p.z0 = (int32_t)z0;
+  p.bx = x1 >= x0 ? (int32_t)(x1 - x0 + 1) : 0;
+  p.by = y1 >= y0 ? (int32_t)(y1 - y0 + 1) : 0;
+  p.bz = z1 >= z0 ? (int32_t)(z1 - z0 + 1) : 0;
+  if (p.bx == 0 || p.by == 0) p.bz = 0;
+  return POL_OK;
+}
+
+int pol_pour_force(pol_ctx* c, const pol_pour_desc* d, void* stream) {
+  if (!c || !d) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_pour_force: buffers not bound");
+  const ForcingArgs f = forcing_args(c);
+  if (!f.force) return fail(c, POL_ERR_STATE, "pol_pour_force: needs the context's force field (POL_FORCE_FIELD)");
+  PourArgs p;
+  int st = pour_args(c, d, p, "pol_pour_force");
+  if (st) return st;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_pour(c->d.storage == POL_F64, f, p, nullptr, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_pour_force");
+}
+
+int pol_pour_phase(pol_ctx* c, const pol_pour_desc* d, float* phi, void* stream) {
+  if (!c || !d || !phi) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_pour_phase: buffers not bound");
+  PourArgs p;
+  int st = pour_args(c, d, p, "pol_pour_phase");
+  if (st) return st;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_pour(c->d.storage == POL_F64, forcing_args(c), p, phi, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_pour_phase");
+}
+
+int pol_density_drive(pol_ctx* c, const float* target, double rate, int half, double max_adjust, double lo, double hi,
+                      void* stream) {
+  if (!c || !target) return POL_ERR_INVALID;
+  if (!c->bound || !c->b.rho) return fail(c, POL_ERR_STATE, "pol_density_drive: buffers not bound");
+  if (!std::isfinite(rate) || !std::isfinite(max_adjust) || !(lo <= hi))
+    return fail(c, POL_ERR_INVALID, "pol_density_drive: invalid parameters");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_density_drive(c->d.storage == POL_F64, forcing_args(c), target, (float)rate, half != 0,
+                             (float)max_adjust, (float)lo, (float)hi, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_density_drive");
+}
+
+int pol_pressure_force(pol_ctx* c, float* pressure_force, int mode, double max_force, double cs2, void* stream) {
+  if (!c || !pressure_force || mode < 0 || mode > 2) return POL_ERR_INVALID;
+  if (!c->bound || !c->b.rho) return fail(c, POL_ERR_STATE, "pol_pressure_force: buffers not bound");
+  const ForcingArgs f = forcing_args(c);
+  if (mode && !f.force)
+    return fail(c, POL_ERR_STATE, "pol_pressure_force: modes 1 and 2 need the context's force field");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_pressure_force(c->d.storage == POL_F64, f, pressure_force, (float)max_force, (float)cs2, mode,
+                              (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_pressure_force");
+}
+
+int pol_filter_blockage(pol_ctx* c, const uint8_t* zone, float* accumulated, float* blockage, void* stream) {
+  if (!c || !zone || !accumulated || !blockage) return POL_ERR_INVALID;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_filter_blockage(zone, accumulated, blockage, c->N, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_filter_blockage");
+}
+
+int pol_filter_block_particles(pol_ctx* c, const uint8_t* zone, float* accumulated, const float* pos, float* vel,
+                               const int32_t* active, int32_t count, double scale, uint32_t seed, void* stream) {
+  if (!c || !zone || !accumulated || count < 0 || (count > 0 && (!pos || !vel || !active))) return POL_ERR_INVALID;
+  if (!(scale > 0.0) || !std::isfinite(scale))
+    return fail(c, POL_ERR_INVALID, "pol_filter_block_particles: scale must be positive");
+  FilterParticleArgs a{};
+  a.zone = zone;
+  a.acc = accumulated;
+  a.pos = pos;
+  a.vel = vel;
+  a.active = active;
+  a.count = (uint32_t)count;
+  a.nx = c->d.nx;
+  a.ny = c->d.ny;
+  a.nz = c->d.nz;
+  a.seed = seed;
+  a.scale = (float)scale;
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_filter_block(a, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_filter_block_particles");
+}
+
 // ---- thermal D3Q7 -----------------------------------------------------------
 static ThermalArgs thermal_args(const pol_ctx* c) {
   ThermalArgs a{};

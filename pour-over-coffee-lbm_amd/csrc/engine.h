@@ -112,6 +112,36 @@ hipError_t launch_mp_phase(bool dbl, const MultiphaseArgs& a, hipStream_t s);
 hipError_t launch_mp_density(bool dbl, const MultiphaseArgs& a, hipStream_t s);
 hipError_t launch_mp_chemical_potential(const MultiphaseArgs& a, float* lap_out, hipStream_t s);
 
+// Body-force producers (forcing.hip).
+struct ForcingArgs {
+  void* force;            // real [3][N] body force (or null)
+  void* rho;              // real [N]
+  const uint32_t* flags;  // cell flags or null (no solids)
+  uint32_t nx, ny, nz, nxy, N;
+  FastDiv dnx, dny;
+};
+struct PourArgs {
+  float cx, cy, radius, zmin, velocity, flow_rate, dt;
+  int32_t height;
+  int32_t x0, y0, z0, bx, by, bz;  // launched box
+};
+struct FilterParticleArgs {
+  const uint8_t* zone;
+  float* acc;
+  const float* pos;  // [3][count]
+  float* vel;        // [3][count]
+  const int32_t* active;
+  uint32_t count, nx, ny, nz, seed;
+  float scale;
+};
+hipError_t launch_pour(bool dbl, const ForcingArgs& f, const PourArgs& p, float* phi, hipStream_t s);
+hipError_t launch_density_drive(bool dbl, const ForcingArgs& f, const float* target, float rate, bool half,
+                                float max_adj, float lo, float hi, hipStream_t s);
+hipError_t launch_pressure_force(bool dbl, const ForcingArgs& f, float* pf, float max_force, float cs2, int mode,
+                                 hipStream_t s);
+hipError_t launch_filter_blockage(const uint8_t* zone, float* acc, float* blockage, uint32_t N, hipStream_t s);
+hipError_t launch_filter_block(const FilterParticleArgs& a, hipStream_t s);
+
 // Thermal D3Q7 state (thermal.hip), dense over every cell.
 struct ThermalArgs {
   const void* gin;

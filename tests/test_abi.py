@@ -126,6 +126,9 @@ int main(void) {
   F(pol_multiphase_buffers, mu) F(pol_multiphase_buffers, phase)
   S(pol_interface_fields) F(pol_interface_fields, grad_phi) F(pol_interface_fields, grad_mu)
   F(pol_interface_fields, normal) F(pol_interface_fields, curvature) F(pol_interface_fields, surface_force)
+  S(pol_pour_desc) F(pol_pour_desc, center_x) F(pol_pour_desc, center_y) F(pol_pour_desc, radius)
+  F(pol_pour_desc, stream_height) F(pol_pour_desc, velocity) F(pol_pour_desc, flow_rate) F(pol_pour_desc, dt)
+  F(pol_pour_desc, height) F(pol_pour_desc, reserved)
   return 0;
 }
 """
@@ -134,7 +137,8 @@ _CTYPES = {"pol_desc": L.PolDesc, "pol_buffers": L.PolBuffers, "pol_particle_des
            "pol_particle_buffers": L.PolParticleBuffers, "pol_adjoint_buffers": L.PolAdjointBuffers,
            "pol_stats_opts": L.PolStatsOpts, "pol_thermal_desc": L.PolThermalDesc,
            "pol_thermal_buffers": L.PolThermalBuffers, "pol_multiphase_desc": L.PolMultiphaseDesc,
-           "pol_multiphase_buffers": L.PolMultiphaseBuffers, "pol_interface_fields": L.PolInterfaceFields}
+           "pol_multiphase_buffers": L.PolMultiphaseBuffers, "pol_interface_fields": L.PolInterfaceFields,
+           "pol_pour_desc": L.PolPourDesc}
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")

@@ -1,0 +1,172 @@
+"""Body-force producers on the device (csrc/forcing.hip through pol_pour_*,
+pol_density_drive, pol_pressure_force, pol_filter_*, and the classes of
+pourover_amd.forcing) against the numpy restatement of the reference's
+pouring, pressure drive and filter dynamics (oracle/forcing.py).
+
+Pressure force, density drive, particle blocking and the accumulated
+particles compare bit for bit.  The pour intensity and the blockage law
+call exp(): the device's expf and numpy's float32 exp may differ in the
+last place, so those fields compare to 3e-7 relative (2.5 ulp), and the set
+of cells touched compares exactly."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import forcing as OF
+from pourover_amd import _lib as L
+from pourover_amd import forcing as FC
+from pourover_amd.engine import SolverConfig
+from pourover_amd.geometry import link_flags
+from pourover_amd.solver import HipLBMSolver
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 22, 20, 24
+F = np.float32
+RT = 3e-7
+
+
+def _same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype, (what, a.shape, b.shape, a.dtype, b.dtype)
+    bad = np.argwhere(a.view(np.int32) != b.view(np.int32))
+    if bad.size:
+        i = tuple(bad[0])
+        raise AssertionError("%s differs at %d places, first %s: %r vs %r" % (what, len(bad), i, a[i], b[i]))
+
+
+def _kind():
+    kind = np.zeros((NZ, NY, NX), np.uint8)
+    kind[:, :, :2] = L.POL_KIND_SOLID
+    kind[16:19, 8:10, 9:12] = L.POL_KIND_SOLID  # inside the pour region
+    return kind
+
+
+def _solver():
+    cfg = SolverConfig(NX, NY, NZ, precision="f32", tau=0.8, periodic=(False, False, False), force_field=True)
+    flags, _ = link_flags(_kind())
+    fluid = HipLBMSolver(cfg, flags)
+    fluid.initialize_fields()
+    return fluid
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _solid():
+    return (_kind() == L.POL_KIND_SOLID).astype(np.uint8)
+
+
+@pytest.mark.parametrize("center", [(10.0, 9.0), (1.0, 18.5)])
+def test_pour_force_and_phase(center):
+    print("case: pour force / phase centre %s" % (center,), flush=True)
+    fluid = _solver()
+    rng = np.random.default_rng(3)
+    bf = rng.normal(0, 1e-3, (3, NZ, NY, NX)).astype(F)
+    bf[0, 17, 12, 10] = F(-0.0)  # a fluid cell of the region: -0 + 0 = +0
+    fluid.ctx.force.copy_(_dev(bf))
+    pour = FC.PrecisePouringSystem(fluid)
+    assert pour.POUR_HEIGHT == NZ - 6 and pour.POUR_DIAMETER_GRID == 8.0 and pour.POUR_VELOCITY == 0.05
+    pour.start_pouring(*center, flow_rate=0.8)
+    pour.apply_pouring_force(fluid.body_force, fluid.solid, 0.5)
+    torch.cuda.synchronize()
+    want = OF.pour_force(bf, _solid(), center[0], center[1], 4.0, pour.POUR_HEIGHT, 0.05, 0.8, 0.5)
+    got = fluid.ctx.force.cpu().numpy()
+    touched = (want != bf)
+    assert np.array_equal(got != bf, touched)
+    np.testing.assert_allclose(got, want, rtol=RT, atol=0)
+    assert np.signbit(got[0, 17, 12, 10]) == np.signbit(want[0, 17, 12, 10])
+    assert touched[2].sum() > 40
+    phi = rng.uniform(-1, 1, (NZ, NY, NX)).astype(F)
+    t = _dev(phi)
+    pour.apply_gradual_phase_change(t, fluid.solid, 0.5)
+    want_phi = OF.pour_phase(phi, _solid(), center[0], center[1], 4.0, pour.POUR_HEIGHT, 0.8, 0.5)
+    got_phi = t.cpu().numpy()
+    assert np.array_equal(got_phi != phi, want_phi != phi)
+    np.testing.assert_allclose(got_phi, want_phi, rtol=RT, atol=1e-7)
+    assert pour.pour_time == F(0.5)
+
+
+def test_spiral_pour_sequence():
+    print("case: spiral pour, 6 calls", flush=True)
+    fluid = _solver()
+    pour = FC.PrecisePouringSystem(fluid)
+    pour.start_pouring(11.0, 10.0, flow_rate=1.0, pattern="spiral")
+    bf = np.zeros((3, NZ, NY, NX), F)
+    t = F(0)
+    for _ in range(6):
+        pour.apply_pouring_force(0.4)
+        t = F(t + F(0.4))
+        x, y = OF.spiral_position(t, 1.0, 5.0, 11.0, 10.0, 8.0, NX, NY)
+        bf = OF.pour_force(bf, _solid(), x, y, 4.0, pour.POUR_HEIGHT, 0.05, 1.0, 0.4)
+    got = fluid.ctx.force.cpu().numpy()
+    assert np.array_equal(got != 0, bf != 0)
+    np.testing.assert_allclose(got, bf, rtol=2e-6, atol=0)
+    info = pour.get_pouring_info()
+    assert info["active"] and info["pattern"] == 1 and abs(info["pour_time"] - 2.4) < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["force", "mixed", "density"])
+def test_pressure_drive(mode):
+    print("case: pressure drive %s" % mode, flush=True)
+    fluid = _solver()
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.9, 1.1, (NZ, NY, NX)).astype(F)
+    rho[4, 4, 4] = F(1e-13)  # rho <= 1e-12: zero force
+    bf = rng.normal(0, 1e-3, (3, NZ, NY, NX)).astype(F)
+    fluid.ctx.rho.copy_(_dev(rho))
+    fluid.ctx.force.copy_(_dev(bf))
+    fluid._macro_valid = True
+    drive = FC.PressureGradientDrive(fluid)
+    _same(drive._target.cpu().numpy(), OF.target_density((NZ, NY, NX)), "target density")
+    solid = _solid()
+    getattr(drive, "activate_%s_drive" % mode)(True)
+    drive.apply(1)
+    torch.cuda.synchronize()
+    if mode == "density":
+        _same(fluid.ctx.rho.cpu().numpy(), OF.density_drive(rho, OF.target_density(rho.shape), solid), "rho")
+        _same(fluid.ctx.force.cpu().numpy(), bf, "body force untouched")
+        return
+    pf = OF.pressure_force(rho, solid)
+    _same(drive._pforce.cpu().numpy(), pf, "pressure force")
+    _same(fluid.ctx.force.cpu().numpy(), OF.accumulate(bf, pf, solid, half=(mode == "mixed")), "body force")
+    st = drive.get_statistics()
+    p = rho[solid == 0].astype(np.float64) * (1.0 / 3.0)
+    assert st["max_pressure"] == pytest.approx(p.max(), rel=1e-7) and st["min_pressure"] == pytest.approx(p.min(), rel=1e-7)
+
+
+def test_filter_dynamics():
+    print("case: filter blockage / particle blocking", flush=True)
+    fluid = _solver()
+    zone = np.zeros((NZ, NY, NX), np.uint8)
+    zone[5] = 1
+    filt = FC.FilterPaperSystem(fluid, zone, seed=11)
+    rng = np.random.default_rng(9)
+    acc = rng.uniform(0, 20, zone.shape).astype(F)
+    blk = rng.uniform(0, 0.5, zone.shape).astype(F)
+    filt._acc.copy_(_dev(acc))
+    filt._blockage.copy_(_dev(blk))
+    filt.update_dynamic_resistance()
+    a, b = OF.filter_blockage(zone, acc, blk)
+    _same(filt._acc.cpu().numpy(), a, "accumulated particles")
+    np.testing.assert_allclose(filt._blockage.cpu().numpy(), b, rtol=RT, atol=0)
+    # particles at distinct columns: half above the paper moving down
+    n = 64
+    s = 0.25
+    cols = rng.choice(NX * NY, n, replace=False)
+    gx, gy = (cols % NX).astype(F), (cols // NX).astype(F)
+    gz = rng.integers(3, 9, n).astype(F)
+    pos = (np.stack([gx, gy, gz]) + F(0.3)) * F(s)
+    vel = rng.normal(0, 0.1, (3, n)).astype(F)
+    active = (rng.random(n) < 0.9).astype(np.int32)
+    acc0 = filt._acc.cpu().numpy()
+    tp, tv, ta = _dev(pos.astype(F)), _dev(vel), _dev(active)
+    filt.block_particles_at_filter(tp, tv, None, ta, scale=s)
+    seed = (11 * 1000003 + 0) & 0xFFFFFFFF
+    wa, wv = OF.block_particles(zone, acc0, pos.astype(F), vel, active, s, seed)
+    _same(tv.cpu().numpy(), wv, "particle velocities")
+    _same(filt._acc.cpu().numpy(), wa, "accumulated particles after blocking")
+    assert (wv[2] != vel[2]).sum() > 5
