@@ -341,8 +341,12 @@ def bench_diagnostics(reps=20, host=True):
 # (7 x 4) and the heat source (4) and writes g (7 x 4); the temperature pass
 # reads g (7 x 4) and writes T (4).  Convection adds T (4, the stencil counted
 # once), u (3 x 4) and the heat-source write (4) to the collide pass (the
-# base heat source is read in place of the heat source).
-THERMAL_BYTES = {False: 92, True: 112}
+# base heat source is read in place of the heat source) and runs the
+# temperature pass every step: 112 B.  Pure diffusion stores T in the next
+# collide pass (+4 B) and runs the temperature pass once per run: 64 B per
+# step + 32 B per run.
+def thermal_bytes(conv: bool, steps: int) -> float:
+    return 112.0 if conv else 64.0 + 32.0 / steps
 
 
 def bench_thermal(n=224, steps=200, warmup=20, cpu=True):
@@ -374,7 +378,7 @@ def bench_thermal(n=224, steps=200, warmup=20, cpu=True):
         stop.record()
         torch.cuda.synchronize()
         sec = start.elapsed_time(stop) / 1e3 / steps
-        algo = THERMAL_BYTES[conv] * n ** 3
+        algo = int(thermal_bytes(conv, steps) * n ** 3)
         out["convection" if conv else "diffusion"] = {
             "ms_per_step": round(sec * 1e3, 4), "mlups": round(n ** 3 / sec / 1e6, 1),
             "algorithmic_bytes_per_step": algo, "achieved_GBs": round(algo / sec / 1e9, 1),

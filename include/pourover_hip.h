@@ -424,10 +424,12 @@ int pol_thermal_bind(pol_ctx *ctx, const pol_thermal_desc *d, const pol_thermal_
 int pol_thermal_set(pol_ctx *ctx, const pol_thermal_desc *d);
 /* g_q = w_q T from the current temperature (thermal_lbm.py:132-170). */
 int pol_thermal_init(pol_ctx *ctx, void *stream);
-/* nsteps thermal steps.  With unstable_out != NULL the last step's stability
- * test (thermal_lbm.py:348-376: T outside [-50, 150] or NaN, a population NaN
- * or |g| > 1000) is copied to *unstable_out (count of failing cells);
- * synchronous then. */
+/* nsteps thermal steps (without convection the temperature of the
+ * intermediate states is formed inside the next step's collide pass, so the
+ * temperature pass runs once per call).  With unstable_out != NULL the last
+ * step's stability test (thermal_lbm.py:348-376: T outside [-50, 150] or
+ * NaN, a population NaN or |g| > 1000) is copied to *unstable_out (count of
+ * failing cells); synchronous then. */
 int pol_thermal_step(pol_ctx *ctx, int nsteps, int32_t *unstable_out, void *stream);
 /* Heat flux q = -k grad T (central differences, interior cells; 0 on the
  * faces) of the current temperature into flux (real [3][cells];

@@ -13,8 +13,10 @@
 // to itself when x - e_q lies outside the box -- the reference's boundary
 // rule, every destination written exactly once.  A second kernel forms the
 // new temperature (the reference's compute_temperature) with the stability
-// test fused in.  Temperature buffers ping-pong, so temperature_old needs no
-// copy.  The arithmetic follows the reference's FP32 expressions operation by
+// test fused in; in a run of pure-diffusion steps each later collide pass
+// stores the temperature of its input state instead, and the second kernel
+// runs once, after the last step.  Temperature buffers ping-pong, so
+// temperature_old needs no copy.  The arithmetic follows the reference's FP32 expressions operation by
 // operation with contraction off:
 //   T = ((((((0 + g0) + g1) + g2) + g3) + g4) + g5) + g6
 //   g* = (g - omega (g - w T)) + (w S) 1
@@ -50,7 +52,10 @@ __device__ inline void decompose(const ThermalArgs& a, uint32_t i, uint32_t& x, 
   y = yz - z * a.ny;
 }
 
-template <typename R, bool CONV>
+// WT: also store T = sum_q g_q of the input state into its temperature slot
+// (a.T) -- the value the previous step's temperature pass would have
+// written, so a run of pure-diffusion steps needs that pass only once.
+template <typename R, bool CONV, bool WT = false>
 __global__ void __launch_bounds__(256) k_thermal_collide(const ThermalArgs a) {
 #pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -66,6 +71,7 @@ __global__ void __launch_bounds__(256) k_thermal_collide(const ThermalArgs a) {
   R T = R(0);
 #pragma unroll
   for (int q = 0; q < TQ; ++q) T = T + g[q];
+  if constexpr (WT) ((R*)a.T)[i] = T;
   R* __restrict__ hs = (R*)a.hs;
   const R* __restrict__ hsb = (const R*)a.hsb;
   R S = hsb ? hsb[i] : hs[i];
@@ -267,6 +273,13 @@ __global__ void __launch_bounds__(256) k_thermal_boundary(const ThermalArgs a, i
 template <typename R>
 static hipError_t thermal_steps(const ThermalArgs& base, void* const g[2], void* const T[2], int& gcur, int& tcur,
                                 int nsteps, hipStream_t s) {
+  // With convection a step's source needs the whole temperature field of its
+  // input state before it starts, so every step runs the temperature pass.
+  // Pure diffusion reads only the cell's own populations: the collide pass of
+  // step n + 1 stores T of its input state (state n + 1) itself, and the
+  // temperature pass (with the stability test that is reported) runs after
+  // the last step only.  The first step leaves the input state's temperature
+  // as the caller set it (it may be a boundary value rather than sum g).
   const dim3 grid((base.N + 255) / 256);
   for (int n = 0; n < nsteps; ++n) {
     ThermalArgs a = base;
@@ -274,11 +287,14 @@ static hipError_t thermal_steps(const ThermalArgs& base, void* const g[2], void*
     a.gout = g[1 - gcur];
     a.T = T[tcur];
     a.Tout = T[1 - tcur];
+    const bool last = n + 1 == nsteps;
     if (a.conv)
       hipLaunchKernelGGL((k_thermal_collide<R, true>), grid, dim3(256), 0, s, a);
-    else
+    else if (n == 0)
       hipLaunchKernelGGL((k_thermal_collide<R, false>), grid, dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_thermal_temperature<R>, grid, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_thermal_collide<R, false, true>), grid, dim3(256), 0, s, a);
+    if (a.conv || last) hipLaunchKernelGGL(k_thermal_temperature<R>, grid, dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     gcur = 1 - gcur;
