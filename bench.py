@@ -13,12 +13,12 @@ Secondary lines in the same JSON object: the all-fluid periodic 224^3 box
 (every cell updated), FP16/BF16 storage speed-up over FP32, the two-way
 coupled step with 50 000 particles, adjoint cost per forward step, field
 diagnostics per call, the thermal D3Q7 step, the phase-field passes, the
-body-force producers, and the CPU baseline (C restatements on the host
-cores, bounded samples).
+body-force producers, the whole main-loop step, and the CPU baseline (C
+restatements on the host cores, bounded samples).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--precision P]
        [--coupled-only | --adjoint-only | --diagnostics-only | --thermal-only | --multiphase-only |
-        --forcing-only]
+        --forcing-only | --main-loop-only]
        [--calibrate-copy]
 """
 from __future__ import annotations
@@ -567,7 +567,7 @@ def bench_forcing(n=224, reps=100, warmup=10):
             ("pour_phase", lambda: pour.apply_gradual_phase_change(phi, None, 1.0), None),
             ("pressure_force_drive", drive.apply_force_drive, 4 * N + 40 * nf),
             ("density_drive", density, 4 * N + 12 * nf),
-            ("filter_blockage", filt.update_dynamic_resistance, N + 16 * nzone)):
+            ("filter_blockage", filt.update_dynamic_resistance, 20 * nzone)):
         t = timed(fn)
         rec = {"ms": round(t * 1e3, 4)}
         if algo:
@@ -575,6 +575,70 @@ def bench_forcing(n=224, reps=100, warmup=10):
                         "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4)})
         out[name] = rec
     del ctx
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_main_loop(n=224, reps=50, warmup=12):
+    """The reference's whole main-loop step (main.py:769-839, from step 11
+    on) on the 224^3 V60 domain, FP32, every part on the device: clear the
+    body force, pour force and phase change, pressure drive (force mode),
+    pre-collision surface tension, the stream-collide step writing rho / u,
+    the filter blockage update and the phase step.  HIP events around
+    `reps` consecutive steps."""
+    import torch
+
+    from pourover_amd import _lib as L
+    from pourover_amd import forcing as FC
+    from pourover_amd import multiphase as MP
+    from pourover_amd import workloads as WL
+    from pourover_amd.engine import HipContext
+
+    wl = WL.v60_config(n, "f32", force_field=True)
+    ctx = HipContext(wl.config, wl.geometry.flags)
+    ctx.init_equilibrium()
+    mp = MP.MultiphaseFlow3D(ctx)
+    mp.init_phase_field()           # the dry cup the boundary set-up leaves (boundary_conditions.py:478-481)
+    mp.standardize_initial_state(force_dry_state=True)
+    pour = FC.PrecisePouringSystem(ctx)
+    pour.start_pouring(pattern="spiral")
+    drive = FC.PressureGradientDrive(ctx)
+    drive.activate_force_drive(True)
+    paper = FC.FilterPaperSystem(ctx, wl.geometry.porous)
+    count = [0]
+
+    def step():
+        count[0] += 1
+        ctx.force.zero_()
+        pour.apply_pouring_force(1.0)
+        pour.apply_gradual_phase_change(mp._phi[mp._cur], None, 1.0)
+        drive.apply(count[0])
+        if count[0] > 10:
+            mp.accumulate_surface_tension_pre_collision()
+        ctx.step(1, L.POL_WANT_MACRO)
+        paper.update_dynamic_resistance()
+        mp.step(count[0], precollision_applied=True)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record()
+    for _ in range(reps):
+        step()
+    b.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    ms = a.elapsed_time(b) / reps
+    water = float((mp._phi[mp._cur] > 0).sum().item())
+    out = {"grid": [n, n, n], "steps": reps, "ms_per_step": round(ms, 4), "wall_ms_per_step": round(wall * 1e3, 4),
+           "fluid_cell_mlups": round((wl.geometry.counts()["fluid"] + wl.geometry.counts()["inlet"] +
+                                      wl.geometry.counts()["outlet"]) / (ms / 1e3) / 1e6, 1),
+           "water_cells_after": int(water),
+           "parts": "clear, pour force + phase, pressure drive, surface tension, stream-collide (+rho/u), "
+                    "filter blockage, phase step"}
+    del mp, ctx
     torch.cuda.empty_cache()
     return out
 
@@ -611,6 +675,8 @@ def main():
                     help="only the 224^3 phase-field measurement (for profiling)")
     ap.add_argument("--forcing-only", action="store_true",
                     help="only the 224^3 body-force producer measurement (for profiling)")
+    ap.add_argument("--main-loop-only", action="store_true",
+                    help="only the 224^3 whole main-loop step measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
@@ -645,6 +711,10 @@ def main():
         log("bench: multiphase 224^3 f32")
         print(json.dumps({"multiphase_224_f32": bench_multiphase(reps=args.steps, warmup=args.warmup, cpu=False)}),
               flush=True)
+        return
+    if args.main_loop_only:
+        log("bench: main-loop step 224^3 f32")
+        print(json.dumps({"main_loop_224_f32": bench_main_loop(reps=args.steps, warmup=args.warmup)}), flush=True)
         return
     if args.forcing_only:
         log("bench: body-force producers 224^3 f32")
@@ -714,6 +784,8 @@ def main():
         out["multiphase_224_f32"] = bench_multiphase(reps=min(args.steps, 100), warmup=args.warmup)
         log("bench: body-force producers 224^3 f32")
         out["forcing_224_f32"] = bench_forcing(reps=min(args.steps, 100), warmup=args.warmup)
+        log("bench: main-loop step 224^3 f32")
+        out["main_loop_224_f32"] = bench_main_loop(reps=min(args.steps, 50))
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

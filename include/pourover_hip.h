@@ -533,9 +533,11 @@ int pol_density_drive(pol_ctx *ctx, const float *target, double rate, int half, 
  * pressure_force (:186-190) or, mode 2, body_force += 0.5 pressure_force
  * (:276-279); mode 0 stores only. */
 int pol_pressure_force(pol_ctx *ctx, float *pressure_force, int mode, double max_force, double cs2, void *stream);
-/* update_dynamic_resistance (filter_paper.py:699-746) on cells with
- * zone == 1 (uint8 [cells]); accumulated / blockage float32 [cells]. */
-int pol_filter_blockage(pol_ctx *ctx, const uint8_t *zone, float *accumulated, float *blockage, void *stream);
+/* update_dynamic_resistance (filter_paper.py:699-746) on the paper cells
+ * listed in cells (ncells cell indices, the cells with filter_zone == 1);
+ * accumulated / blockage float32 [cells of the grid]. */
+int pol_filter_blockage(pol_ctx *ctx, const uint32_t *cells, int32_t ncells, float *accumulated, float *blockage,
+                        void *stream);
 /* block_particles_at_filter (filter_paper.py:616-697): float32 SoA
  * positions / velocities [3][count], int32 active; grid cell = int(pos /
  * scale).  The horizontal kick draws from a counter-based generator keyed

@@ -10,7 +10,8 @@ drive, pressure gradient force and its accumulation) and
 src/physics/filter_paper.py:616-746 (particle blocking, dynamic blockage).
 Arrays are in the device order of the engine ([nz, ny, nx] scalars,
 [3, nz, ny, nx] vectors) and float32; expressions keep the reference's
-operation order (numpy performs no contraction).  Python-float constants
+operation order (numpy performs no contraction); exp is formed in float64
+and rounded to float32, as the device does.  Python-float constants
 the reference writes as expressions of Python floats are formed in double
 and rounded once.
 """
@@ -19,6 +20,11 @@ from __future__ import annotations
 import numpy as np
 
 F = np.float32
+
+
+def exp32(x):
+    """float32 exp formed in double and rounded once (the device's form)."""
+    return np.exp(np.asarray(x, dtype=np.float64)).astype(F)
 
 
 # ---- pouring ---------------------------------------------------------------------------
@@ -44,8 +50,8 @@ def pour_intensity(shape, cx, cy, radius, height, stream_height=4.0) -> np.ndarr
     inside = (d <= rad) & (k <= height) & (k.astype(F) >= F(height - stream_height))
     r = d / rad
     with np.errstate(over="ignore", under="ignore"):
-        inten = np.exp(F(-0.5) * (r * r))
-        decay = np.exp((-(height - k)).astype(F) / F(2.0))
+        inten = exp32(F(-0.5) * (r * r))
+        decay = exp32((-(height - k)).astype(F) / F(2.0))
     return np.where(inside, inten * decay, F(0)).astype(F)
 
 
@@ -131,7 +137,7 @@ def accumulate(body_force, pf, solid, half=False):
 def filter_blockage(zone, acc, blockage):
     """update_dynamic_resistance (filter_paper.py:699-746): (acc, blockage)."""
     z = zone == 1
-    nb = F(0.9) * (F(1.0) - np.exp(F(-0.1) * acc))
+    nb = F(0.9) * (F(1.0) - exp32(F(-0.1) * acc))
     b = np.where(z, F(0.95) * blockage + F(0.05) * nb, blockage).astype(F)
     a = np.where(z, acc * F(0.999), acc).astype(F)
     return a, b

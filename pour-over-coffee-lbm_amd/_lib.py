@@ -222,7 +222,7 @@ SIGNATURES = {
     "pol_pour_phase": (C.c_int, [_P, C.POINTER(PolPourDesc), _P, _P]),
     "pol_density_drive": (C.c_int, [_P, _P, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, _P]),
     "pol_pressure_force": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, _P]),
-    "pol_filter_blockage": (C.c_int, [_P, _P, _P, _P, _P]),
+    "pol_filter_blockage": (C.c_int, [_P, _P, C.c_int32, _P, _P, _P]),
     "pol_filter_block_particles": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_uint32, _P]),
 }
 

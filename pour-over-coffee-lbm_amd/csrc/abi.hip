@@ -790,10 +790,13 @@ int pol_pressure_force(pol_ctx* c, float* pressure_force, int mode, double max_f
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_pressure_force");
 }
 
-int pol_filter_blockage(pol_ctx* c, const uint8_t* zone, float* accumulated, float* blockage, void* stream) {
-  if (!c || !zone || !accumulated || !blockage) return POL_ERR_INVALID;
+int pol_filter_blockage(pol_ctx* c, const uint32_t* cells, int32_t ncells, float* accumulated, float* blockage,
+                        void* stream) {
+  if (!c || ncells < 0 || (ncells > 0 && !cells) || !accumulated || !blockage) return POL_ERR_INVALID;
+  if ((uint32_t)ncells > c->N)
+    return fail(c, POL_ERR_INVALID, "pol_filter_blockage: more paper cells than grid cells");
   hipError_t e = hipSetDevice(c->d.device);
-  if (e == hipSuccess) e = launch_filter_blockage(zone, accumulated, blockage, c->N, (hipStream_t)stream);
+  if (e == hipSuccess) e = launch_filter_blockage(cells, (uint32_t)ncells, accumulated, blockage, (hipStream_t)stream);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_filter_blockage");
 }
 

@@ -139,7 +139,7 @@ hipError_t launch_density_drive(bool dbl, const ForcingArgs& f, const float* tar
                                 float max_adj, float lo, float hi, hipStream_t s);
 hipError_t launch_pressure_force(bool dbl, const ForcingArgs& f, float* pf, float max_force, float cs2, int mode,
                                  hipStream_t s);
-hipError_t launch_filter_blockage(const uint8_t* zone, float* acc, float* blockage, uint32_t N, hipStream_t s);
+hipError_t launch_filter_blockage(const uint32_t* cells, uint32_t ncells, float* acc, float* blockage, hipStream_t s);
 hipError_t launch_filter_block(const FilterParticleArgs& a, hipStream_t s);
 
 // Thermal D3Q7 state (thermal.hip), dense over every cell.

@@ -21,6 +21,13 @@
 
 namespace pol {
 
+// float32 exp formed in double and rounded once: the correctly rounded
+// value except when the double result lies within 2^-29 of a float rounding
+// boundary, and the same value numpy gives for float64 exp rounded to
+// float32 -- so the pour intensity and the blockage law are reproducible
+// bit for bit by a host restatement.
+__device__ inline float exp_rn(float x) { return (float)exp((double)x); }
+
 __device__ inline bool fc_solid(const ForcingArgs& f, uint32_t c) {
   return f.flags && ((f.flags[c] >> POL_KIND_SHIFT) & POL_KIND_MASK) == POL_KIND_SOLID;
 }
@@ -34,8 +41,8 @@ __device__ inline float pour_intensity(const PourArgs& p, int i, int j, int k) {
   float total = 0.0f;
   if (d <= p.radius && k <= p.height && (float)k >= p.zmin) {
     const float r = d / p.radius;
-    const float intensity = expf(-0.5f * (r * r));
-    const float decay = expf((float)(-(p.height - k)) / 2.0f);
+    const float intensity = exp_rn(-0.5f * (r * r));
+    const float decay = exp_rn((float)(-(p.height - k)) / 2.0f);
     total = intensity * decay;
   }
   return total;
@@ -145,14 +152,15 @@ __global__ void __launch_bounds__(256) k_pressure_force(const ForcingArgs f, flo
   }
 }
 
-// filter_paper.py:699-746 on filter cells
-__global__ void __launch_bounds__(256) k_filter_blockage(const uint8_t* __restrict__ zone, float* __restrict__ acc,
-                                                         float* __restrict__ blockage, uint32_t N) {
+// filter_paper.py:699-746 on the listed paper cells
+__global__ void __launch_bounds__(256) k_filter_blockage(const uint32_t* __restrict__ cells, uint32_t ncells,
+                                                         float* __restrict__ acc, float* __restrict__ blockage) {
 #pragma clang fp contract(off)
-  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
-  if (c >= N || zone[c] != 1) return;
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= ncells) return;
+  const uint32_t c = cells[t];
   const float a = acc[c];
-  const float nb = 0.9f * (1.0f - expf(-0.1f * a));
+  const float nb = 0.9f * (1.0f - exp_rn(-0.1f * a));
   blockage[c] = 0.95f * blockage[c] + 0.05f * nb;
   acc[c] = a * 0.999f;
 }
@@ -231,8 +239,9 @@ hipError_t launch_pressure_force(bool dbl, const ForcingArgs& f, float* pf, floa
   return hipGetLastError();
 }
 
-hipError_t launch_filter_blockage(const uint8_t* zone, float* acc, float* blockage, uint32_t N, hipStream_t s) {
-  hipLaunchKernelGGL(k_filter_blockage, dim3((N + 255) / 256), dim3(256), 0, s, zone, acc, blockage, N);
+hipError_t launch_filter_blockage(const uint32_t* cells, uint32_t ncells, float* acc, float* blockage, hipStream_t s) {
+  if (ncells == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_filter_blockage, dim3((ncells + 255) / 256), dim3(256), 0, s, cells, ncells, acc, blockage);
   return hipGetLastError();
 }
 

@@ -283,6 +283,8 @@ class FilterPaperSystem:
         dev = self.ctx.device
         z = np.ascontiguousarray(np.asarray(filter_zone) != 0).astype(np.uint8)
         self._zone = torch.from_numpy(z).to(dev)
+        # the paper cells, listed once: the blockage update touches only them
+        self._cells = torch.from_numpy(np.flatnonzero(z.ravel() == 1).astype(np.int32)).to(dev)
         self._acc = torch.zeros(z.shape, dtype=torch.float32, device=dev)
         self._blockage = torch.zeros(z.shape, dtype=torch.float32, device=dev)
         self.seed = int(seed)
@@ -301,8 +303,8 @@ class FilterPaperSystem:
         return DeviceField(self._blockage, "scalar")
 
     def update_dynamic_resistance(self) -> None:
-        L.check(self.lib.pol_filter_blockage(self.ctx.handle, self._zone.data_ptr(), self._acc.data_ptr(),
-                                             self._blockage.data_ptr(), L.stream_handle()),
+        L.check(self.lib.pol_filter_blockage(self.ctx.handle, self._cells.data_ptr(), int(self._cells.numel()),
+                                             self._acc.data_ptr(), self._blockage.data_ptr(), L.stream_handle()),
                 self.ctx.handle, "pol_filter_blockage")
 
     def block_particles_at_filter(self, positions, velocities, radii=None, active=None, count=None,
