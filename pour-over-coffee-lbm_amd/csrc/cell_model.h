@@ -277,10 +277,32 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
 //     anti = 3 c omega rho e.u + 3 c cf e.F.
 // Mass and momentum stay conserved to rounding (the identities above hold
 // term by term).  Physics, branches and outputs are those of collide_cell.
-template <bool FULL>
+// FP16 operands straight from the loaded 16-bit words (low half of a VGPR):
+// v_fma_mix_f32 widens them exactly inside the FMA, so a + b, a - b and
+// k a + c round once, exactly as the same operations on converted values --
+// without the 19 separate conversions.
+__device__ inline float mix_add(uint32_t a, uint32_t b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ inline float mix_sub(uint32_t a, uint32_t b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ inline float mix_fma(float k, uint32_t a, float c) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,1,0]" : "=v"(r) : "v"(k), "v"(a), "v"(c));
+  return r;
+}
+
+// MIX: the populations 1..18 come as FP16 words in h (d[0] decoded); the
+// pair sums, differences and the relaxation read them through mix_*.
+template <bool FULL, bool MIX = false>
 __device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, bool porous, bool les, float K,
                                                  float fx, float fy, float fz, float bx, float by, float bz, float B,
-                                                 const Phys<float>& P) {
+                                                 const Phys<float>& P, const uint32_t (&h)[Q]) {
   using R = float;
   CellOut<R> o;
   if (FULL && kind == POL_KIND_INLET) {
@@ -297,9 +319,15 @@ __device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, b
     constexpr int p = decltype(pc)::value;
     constexpr int a = PAIR_A(p);
     constexpr int b = OPP(a);
-    sp[p] = d[a] + d[b];
+    R A;
+    if constexpr (MIX) {
+      sp[p] = mix_add(h[a], h[b]);
+      A = mix_sub(h[a], h[b]);
+    } else {
+      sp[p] = d[a] + d[b];
+      A = d[a] - d[b];
+    }
     s36 += sp[p];
-    const R A = d[a] - d[b];
     eacc<EX(a)>(mx, A);
     eacc<EY(a)>(my, A);
     eacc<EZ(a)>(mz, A);
@@ -392,8 +420,13 @@ __device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, b
       anti += (c * A2) * eF;
     }
     const R sym = eu * t + c * S0;
-    d[a] = keep * d[a] + (sym + anti);
-    d[b] = keep * d[b] + (sym - anti);
+    if constexpr (MIX) {
+      d[a] = mix_fma(keep, h[a], sym + anti);
+      d[b] = mix_fma(keep, h[b], sym - anti);
+    } else {
+      d[a] = keep * d[a] + (sym + anti);
+      d[b] = keep * d[b] + (sym - anti);
+    }
   });
   return o;
 }

@@ -48,7 +48,8 @@ struct CompactRows {
 // Forward (pull) sources of a cell: link q reads slot rb[(1 - e_y) + 3 (1 - e_z)]
 // + x - e_x of plane q, or D*_{opp q} at its own slot when bit q of the cell
 // flags is set.
-template <typename S, bool A32, bool XF, typename Use>
+// RAW: hand use() the stored value itself instead of its decoded real.
+template <typename S, bool A32, bool XF, bool RAW = false, typename Use>
 __device__ __forceinline__ void compact_sources(const S* __restrict__ fin, uint32_t PS, const int (&rb)[9], uint32_t x,
                                                 uint32_t nx, bool px, uint32_t own, uint32_t fl, Use&& use) {
   using C = Codec<S>;
@@ -83,7 +84,9 @@ __device__ __forceinline__ void compact_sources(const S* __restrict__ fin, uint3
       off = (bounce & m) | (off & ~m);
     }
     const S* base = fin + (AI)lo * PS;
-    use(qc, C::decode(*((const S*)((const char*)base + off) - EX(q))));
+    const S v = *((const S*)((const char*)base + off) - EX(q));
+    if constexpr (RAW) use(qc, v);
+    else use(qc, C::decode(v));
   });
 }
 

@@ -85,12 +85,25 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   if constexpr (SPARSE) own = slot_clamp(rb[4] + sx, PS);
 
   R f[Q];  // scaled deviations 36 (f - w)
+  // FP16 compact launches keep the loaded words and let the collision widen
+  // them inside its first operations (collide_cell_lp<.., true>)
+  constexpr bool MIX = SPARSE && std::is_same_v<S, _Float16>;
+  uint32_t h[Q];
   CellOut<R> o{R(0), R(0), R(0), R(0)};
   if (active) {
     if constexpr (SPARSE) {
-      if (kind != POL_KIND_INLET)
-        compact_sources<S, A32, XF>(fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl,
-                                    [&](auto qc, R v) { f[decltype(qc)::value] = v; });
+      if (kind != POL_KIND_INLET) {
+        if constexpr (MIX) {
+          compact_sources<S, A32, XF, true>(fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl,
+                                            [&](auto qc, S v) {
+                                              h[decltype(qc)::value] = __builtin_bit_cast(uint16_t, v);
+                                            });
+          f[0] = (R)__builtin_bit_cast(_Float16, (uint16_t)h[0]);
+        } else {
+          compact_sources<S, A32, XF>(fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl,
+                                      [&](auto qc, R v) { f[decltype(qc)::value] = v; });
+        }
+      }
     } else if (!FULL || kind != POL_KIND_INLET) {
       // neighbour offsets; non-periodic axes clamp to the cell's own plane
       // (zero-gradient for open-boundary cells; walls use bounce-back bits)
@@ -140,7 +153,7 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     // moments-only passes use the same velocity definition the collision uses
     const bool les_on = STORE && les;
     if constexpr (sizeof(S) == 2)
-      o = collide_cell_lp<FULL>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P);
+      o = collide_cell_lp<FULL, MIX>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P, h);
     else
       o = collide_cell<R, FULL>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P);
   }
