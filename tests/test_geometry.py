@@ -44,7 +44,7 @@ def test_digest_224(golden):
     ref = golden["geometry"]["224"]
     assert g.counts() == ref["counts"]
     assert g.digest() == ref["sha256"]
-    # BASELINE geometry: outlet hole radius 8 -> outlet plane = discrete disk r <= 8 around (111.5, 111.5)
+    # the 224^3 V60 cup: outlet hole radius 8 -> outlet plane = discrete disk r <= 8 around (111.5, 111.5)
     yy, xx = np.mgrid[0:224, 0:224]
     disk = ((xx - 111.5) ** 2 + (yy - 111.5) ** 2) <= 64.0
     assert g.counts()["outlet"] == int(disk.sum())

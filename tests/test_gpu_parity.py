@@ -2,7 +2,7 @@
 
 Every call goes through the C ABI (engine.HipContext -> libpourover_hip.so).
 Bars: FP32 storage/arithmetic within 1e-5 relative L2 on rho and u of the
-FP64 oracle, FP64 within 1e-12.
+FP64 oracle (the FP32 parity tolerance of the path), FP64 within 1e-12.
 """
 from __future__ import annotations
 
