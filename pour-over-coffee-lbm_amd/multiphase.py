@@ -82,25 +82,23 @@ class MultiphaseFlow3D:
         L.check(self.lib.pol_multiphase_bind(self.ctx.handle, C.byref(d), C.byref(b)), self.ctx.handle,
                 "pol_multiphase_bind")
         self._cur = 0
-        self._fields_phi = None  # the phi the stored interface fields belong to (id, step)
         self._steps = 0
 
     # ---- fields (reference index order [x, y, z]) ------------------------------------
     def _interface_fields(self) -> None:
-        key = (self._cur, self._steps)
-        if self._fields_phi == key:
-            return
+        """Form grad_phi, grad_mu, normal, curvature and surface_force of the
+        current phi and mu (on every read: phi may have been changed in place,
+        e.g. by the pouring phase change)."""
         out = L.PolInterfaceFields()
         out.grad_phi, out.grad_mu = self._grad_phi.data_ptr(), self._grad_mu.data_ptr()
         out.normal, out.curvature = self._normal.data_ptr(), self._curv.data_ptr()
         out.surface_force = self._sforce.data_ptr()
         L.check(self.lib.pol_multiphase_interface(self.ctx.handle, 0, C.byref(out), L.stream_handle()),
                 self.ctx.handle, "pol_multiphase_interface")
-        self._fields_phi = key
 
     @property
     def phi(self) -> DeviceField:
-        return DeviceField(self._phi[self._cur], "scalar", after_write=self._phi_written)
+        return DeviceField(self._phi[self._cur], "scalar")
 
     @property
     def phi_new(self) -> DeviceField:
@@ -115,12 +113,9 @@ class MultiphaseFlow3D:
             c = self._phi[self._cur]
             self._phi_new[1:-1, 1:-1, 1:-1].copy_(c[1:-1, 1:-1, 1:-1])
 
-    def _phi_written(self) -> None:
-        self._fields_phi = None
-
     @property
     def mu(self) -> DeviceField:
-        return DeviceField(self._mu, "scalar", after_write=self._phi_written)
+        return DeviceField(self._mu, "scalar")
 
     @property
     def laplacian_phi(self) -> DeviceField:
@@ -156,13 +151,11 @@ class MultiphaseFlow3D:
     def init_phase_field(self) -> None:
         """Every cell air (phi = -1)."""
         self._phi[self._cur].fill_(-1.0)
-        self._phi_written()
 
     def _set_dry_initial_state(self) -> None:
         fluid = ~self._solid()
         self._phi[self._cur].masked_fill_(fluid, -1.0)
         self._phi_new.masked_fill_(fluid, -1.0)
-        self._phi_written()
 
     def update_density_from_phase(self) -> None:
         L.check(self.lib.pol_multiphase_density(self.ctx.handle, L.stream_handle()), self.ctx.handle,
@@ -171,10 +164,8 @@ class MultiphaseFlow3D:
     def compute_chemical_potential(self) -> None:
         L.check(self.lib.pol_multiphase_chemical_potential(self.ctx.handle, self._lap.data_ptr(), L.stream_handle()),
                 self.ctx.handle, "pol_multiphase_chemical_potential")
-        self._fields_phi = None
 
     def compute_gradients(self) -> None:
-        self._fields_phi = None
         self._interface_fields()
 
     compute_curvature = compute_gradients
