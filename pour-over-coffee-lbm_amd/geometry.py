@@ -23,7 +23,7 @@ from dataclasses import dataclass, asdict
 import numpy as np
 
 from . import _lib as L
-from .lattice import E, OPP, Q
+from .lattice import E, Q
 
 
 @dataclass(frozen=True)

@@ -9,15 +9,14 @@ protocol verbs map onto it as documented per method.
 """
 from __future__ import annotations
 
-import math
-from typing import Any, Dict, Optional, Protocol, Sequence, Tuple, runtime_checkable
+from typing import Any, Dict, Optional, Protocol, Tuple, runtime_checkable
 
 import numpy as np
 import torch
 
 from . import _lib as L
 from .backends import ComputeBackend, HipBackend, create_backend
-from .engine import HipContext, SolverConfig, reduction_dict
+from .engine import HipContext, SolverConfig
 from .fields import DeviceField
 from .lattice import OPP
 

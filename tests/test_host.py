@@ -18,7 +18,7 @@ from pourover_amd import fields as F
 from pourover_amd import geometry as G
 from pourover_amd import workloads as WL
 from pourover_amd.adjoint import auto_checkpoint
-from pourover_amd.engine import CHUNK_CELLS, WRITE_GRANULE, HipContext, SolverConfig
+from pourover_amd.engine import CHUNK_CELLS, WRITE_GRANULE, HipContext
 from pourover_amd.solver import HipLBMSolver, LBMSolverProtocol, ParticleCoupledLBMProtocol
 
 ROOT = Path(__file__).resolve().parents[1]

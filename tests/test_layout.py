@@ -8,7 +8,7 @@ import pytest
 
 from pourover_amd import _lib as L
 from pourover_amd import geometry as G
-from pourover_amd.lattice import E, OPP, Q
+from pourover_amd.lattice import E, Q
 from pourover_amd.layout import RECORD_WORDS, build_compact_layout
 
 
