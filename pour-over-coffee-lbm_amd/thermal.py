@@ -17,6 +17,8 @@ reference's [x, y, z] order.  There is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import time
+from dataclasses import dataclass
 from typing import Optional, Tuple
 
 import numpy as np
@@ -329,37 +331,160 @@ class ThermalLBM:
             self._hs.copy_(src)
 
 
-class ThermalFluidCoupledSolver:
-    """thermal_fluid_coupled.py:143-235: fluid step, velocity hand-over every
-    coupling_frequency steps, heat source reset to its base, thermal
-    sub-cycles."""
+@dataclass
+class CouplingConfig:
+    """thermal_fluid_coupled.py:35-42."""
 
-    def __init__(self, fluid_solver, thermal: ThermalLBM, coupling_frequency: int = 1, thermal_subcycles: int = 1,
-                 base_heat_source: Optional[np.ndarray] = None):
+    coupling_frequency: int = 1
+    velocity_smoothing: bool = True
+    thermal_subcycles: int = 1
+    enable_diagnostics: bool = True
+    max_coupling_error: float = 1e6
+
+
+class ThermalFluidCoupledSolver:
+    """thermal_fluid_coupled.py:44-460: a fluid solver and a ThermalLBM that
+    shares its context; each step runs the fluid step, hands the velocity
+    over every coupling_frequency steps, resets the heat source to its base
+    and runs the thermal sub-cycles (:143-209).  With one sub-cycle the reset
+    runs inside the thermal step (POL_THERMAL_RESET_SOURCE) instead of as a
+    copy.  The reference builds its own 224^3 solvers; here they may be
+    passed in (fluid_solver / thermal_solver) or are built on `grid`."""
+
+    def __init__(self, coupling_config: Optional[CouplingConfig] = None, thermal_diffusivity: float = 1.6e-7, *,
+                 fluid_solver=None, thermal_solver: Optional[ThermalLBM] = None, grid=(224, 224, 224),
+                 precision: str = "f32"):
+        self.coupling_config = coupling_config or CouplingConfig()
+        if fluid_solver is None:
+            from .solver import HipLBMSolver
+
+            fluid_solver = HipLBMSolver(SolverConfig(*grid, precision=precision, periodic=(False, False, False)))
         self.fluid_solver = fluid_solver
-        self.thermal_solver = thermal
-        self.coupling_frequency = coupling_frequency
-        self.thermal_subcycles = thermal_subcycles
+        self.thermal_solver = thermal_solver if thermal_solver is not None else \
+            ThermalLBM(thermal_diffusivity, solver=fluid_solver)
         self.thermal_solver.enable_convection_coupling(True)
-        self.base_heat_source = torch.zeros_like(thermal._hs)
-        if base_heat_source is not None:
-            self.base_heat_source.copy_(torch.from_numpy(
-                np.ascontiguousarray(base_heat_source.astype(np.float32).transpose(2, 1, 0))))
-        # one sub-cycle: the reset to the base source runs inside the thermal step
-        self.fused_reset = thermal_subcycles == 1
+        if hasattr(fluid_solver, "enable_thermal_coupling_output"):
+            fluid_solver.enable_thermal_coupling_output(True)
+        self.base_heat_source = torch.zeros_like(self.thermal_solver._hs)
+        self.fused_reset = self.coupling_config.thermal_subcycles == 1
         if self.fused_reset:
             self.thermal_solver.use_base_heat_source(self.base_heat_source)
         self.coupling_step = 0
+        self.is_initialized = False
+        self._reset_stats()
+
+    def _reset_stats(self) -> None:
+        self.performance_stats = {"fluid_time": 0.0, "thermal_time": 0.0, "coupling_time": 0.0, "total_steps": 0}
+
+    def set_base_heat_source(self, base_heat_source: Optional[np.ndarray]) -> None:
+        """The base source [nx, ny, nz] every step starts from (zeros if None)."""
+        if base_heat_source is None:
+            self.base_heat_source.zero_()
+        else:
+            b = np.asarray(base_heat_source)
+            if b.shape != (self.thermal_solver.nx, self.thermal_solver.ny, self.thermal_solver.nz):
+                raise ValueError("heat source shape %s does not match the grid" % (b.shape,))
+            self.base_heat_source.copy_(torch.from_numpy(np.ascontiguousarray(b.astype(np.float32).transpose(2, 1, 0))))
+            self.thermal_solver.reset_heat_source_to_base(self.base_heat_source)
+        if self.fused_reset:
+            self.thermal_solver.use_base_heat_source(self.base_heat_source)
+
+    def initialize_system(self, fluid_initial_conditions: dict, thermal_initial_conditions: dict,
+                          base_heat_source: Optional[np.ndarray] = None) -> None:
+        """thermal_fluid_coupled.py:100-141."""
+        self.fluid_solver.init_fields()
+        self.thermal_solver.complete_initialization(
+            T_initial=thermal_initial_conditions.get("T_initial", 25.0),
+            T_hot_region=thermal_initial_conditions.get("T_hot_region", 93.0),
+            hot_region_height=thermal_initial_conditions.get("hot_region_height", 20))
+        self.set_base_heat_source(base_heat_source)
+        self.is_initialized = True
 
     def step(self) -> bool:
+        """thermal_fluid_coupled.py:143-209."""
+        if not self.is_initialized:
+            return False
+        cfg = self.coupling_config
+        t0 = time.time()
         self.fluid_solver.step()
-        if self.coupling_step % self.coupling_frequency == 0:
+        t1 = time.time()
+        self.performance_stats["fluid_time"] += t1 - t0
+        if self.coupling_step % cfg.coupling_frequency == 0:
             get = getattr(self.fluid_solver, "get_velocity_field_for_thermal_coupling", None)
             self.thermal_solver.set_velocity_field(get() if get is not None else self.fluid_solver.u)
+            self.performance_stats["coupling_time"] += time.time() - t1
+        t2 = time.time()
         if not self.fused_reset:
             self.thermal_solver.reset_heat_source_to_base(self.base_heat_source)
-        for _ in range(self.thermal_subcycles):
+        for _ in range(cfg.thermal_subcycles):
             if not self.thermal_solver.step():
                 return False
+        self.performance_stats["thermal_time"] += time.time() - t2
+        if cfg.enable_diagnostics and not self._check_coupling_stability():
+            return False
         self.coupling_step += 1
+        self.performance_stats["total_steps"] += 1
         return True
+
+    def _check_coupling_stability(self) -> bool:
+        """thermal_fluid_coupled.py:237-262 (the velocity bound on the device)."""
+        tmin, tmax, _ = self.thermal_solver.get_temperature_stats()
+        if tmax > 150.0 or tmin < -10.0:
+            return False
+        if abs(tmax - tmin) > self.coupling_config.max_coupling_error:
+            return False
+        u = self.thermal_solver._vel
+        return not bool((torch.linalg.vector_norm(u, dim=0) > 1.0).any())
+
+    def get_coupling_diagnostics(self) -> dict:
+        """thermal_fluid_coupled.py:264-304."""
+        ps = self.performance_stats
+        if ps["total_steps"] == 0:
+            return {"status": "not_started"}
+        tmin, tmax, tavg = self.thermal_solver.get_temperature_stats()
+        total = ps["fluid_time"] + ps["thermal_time"] + ps["coupling_time"]
+        frac = (lambda v: v / total if total > 0 else 0)
+        return {"coupling_step": self.coupling_step,
+                "thermal_stats": {"T_min": float(tmin), "T_max": float(tmax), "T_avg": float(tavg),
+                                  "thermal_diffusivity": self.thermal_solver.get_effective_thermal_diffusivity()},
+                "performance": {"fluid_fraction": frac(ps["fluid_time"]), "thermal_fraction": frac(ps["thermal_time"]),
+                                "coupling_fraction": frac(ps["coupling_time"]),
+                                "steps_per_second": ps["total_steps"] / total if total > 0 else 0},
+                "coupling_config": {"frequency": self.coupling_config.coupling_frequency,
+                                    "thermal_subcycles": self.coupling_config.thermal_subcycles,
+                                    "convection_enabled": self.thermal_solver.enable_convection}}
+
+    def save_coupling_state(self, step_num: int) -> dict:
+        """thermal_fluid_coupled.py:306-333 (host copies, reference index order)."""
+        return {"step": step_num, "velocity": self.fluid_solver.u.to_numpy(), "density": self.fluid_solver.rho.to_numpy(),
+                "temperature": self.thermal_solver.temperature.to_numpy(),
+                "heat_flux": self.thermal_solver.heat_flux.to_numpy()}
+
+    def reset_coupling_system(self) -> None:
+        """thermal_fluid_coupled.py:335-355."""
+        if hasattr(self.fluid_solver, "init_fields"):
+            self.fluid_solver.init_fields()
+        self.thermal_solver.reset()
+        self.coupling_step = 0
+        self.is_initialized = False
+        self._reset_stats()
+
+    # fluid-solver fields seen through the coupled solver (thermal_fluid_coupled.py:357-415)
+    @property
+    def solid(self):
+        return self.fluid_solver.solid
+
+    @property
+    def rho(self):
+        return self.fluid_solver.rho
+
+    @property
+    def phase(self):
+        return getattr(self.fluid_solver, "phase", None)
+
+    @property
+    def u(self):
+        return self.fluid_solver.u
+
+    def get_temperature_field(self):
+        return self.thermal_solver.temperature

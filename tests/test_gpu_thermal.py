@@ -207,7 +207,10 @@ def test_coupled_step_matches_oracle(subcycles):
     T, hs, _ = _fields(9, np.float32)
     th.initialize_from(T)
     base = hs.copy()
-    cs = TH.ThermalFluidCoupledSolver(fluid, th, thermal_subcycles=subcycles, base_heat_source=base)
+    cs = TH.ThermalFluidCoupledSolver(TH.CouplingConfig(thermal_subcycles=subcycles), fluid_solver=fluid,
+                                      thermal_solver=th)
+    cs.set_base_heat_source(base)
+    cs.is_initialized = True
     assert cs.fused_reset == (subcycles == 1)
     st = {"g": OT.equilibrium(T), "T": T, "heat_source": base}
     for _ in range(2):
@@ -220,3 +223,26 @@ def test_coupled_step_matches_oracle(subcycles):
     _assert_same(th.temperature.to_numpy(), st["T"], "coupled T")
     _assert_same(th.heat_source.to_numpy(), st["heat_source"], "coupled heat source")
     assert np.abs(u).max() > 0
+
+
+def test_coupled_solver_reference_interface():
+    """thermal_fluid_coupled.py:58-355: the solver builds its own fluid and
+    thermal solvers, initialize_system, steps, diagnostics, state, reset."""
+    print("case: coupled solver interface", flush=True)
+    cs = TH.ThermalFluidCoupledSolver(grid=(NX, NY, NZ))
+    assert not cs.step()                                   # not initialised
+    assert cs.get_coupling_diagnostics() == {"status": "not_started"}
+    base = np.full((NX, NY, NZ), 0.01, np.float32)
+    cs.initialize_system({}, {"T_initial": 25.0, "T_hot_region": 90.0, "hot_region_height": 4}, base)
+    for _ in range(3):
+        assert cs.step()
+    d = cs.get_coupling_diagnostics()
+    # over-relaxation (omega = 1.73) undershoots next to the 90 / 25 step; the
+    # coupled stability bounds are -10 and 150
+    assert d["coupling_step"] == 3 and -10.0 < d["thermal_stats"]["T_min"] < d["thermal_stats"]["T_max"] < 150.0
+    assert d["coupling_config"]["convection_enabled"]
+    st = cs.save_coupling_state(3)
+    assert st["temperature"].shape == (NX, NY, NZ) and st["velocity"].shape == (NX, NY, NZ, 3)
+    _assert_same(cs.thermal_solver.heat_source.to_numpy(), base, "heat source = base after the fused reset")
+    cs.reset_coupling_system()
+    assert cs.coupling_step == 0 and not cs.is_initialized
