@@ -219,6 +219,23 @@ class MultiphaseFlow3D:
         self._cur ^= 1
         self._steps += 1
 
+    # The reference's step runs these four in a row (multiphase_3d.py:405-414);
+    # on the device they are one pass, run by the first of them.
+    def update_phase_field_cahn_hilliard(self) -> None:
+        """multiphase_3d.py:151-197 together with apply_phase_separation,
+        copy_phase_field and update_density_from_phase (one pass)."""
+        self._ensure_velocity()
+        L.check(self.lib.pol_multiphase_step(self.ctx.handle, L.stream_handle()), self.ctx.handle,
+                "pol_multiphase_step")
+        self._cur ^= 1
+        self._steps += 1
+
+    def apply_phase_separation(self) -> None:
+        """Applied inside update_phase_field_cahn_hilliard's pass (multiphase_3d.py:383-401)."""
+
+    def copy_phase_field(self) -> None:
+        """The phase field ping-pongs: the pass already made phi current (multiphase_3d.py:418-422)."""
+
     def get_interface_statistics(self) -> dict:
         """multiphase_3d.py:301-322 (float64 reductions on the device)."""
         self._interface_fields()
