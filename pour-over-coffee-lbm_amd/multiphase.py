@@ -213,11 +213,7 @@ class MultiphaseFlow3D:
         """multiphase_3d.py:389-414."""
         if (not precollision_applied) and step_count > 10:
             self.apply_surface_tension()
-        self._ensure_velocity()
-        L.check(self.lib.pol_multiphase_step(self.ctx.handle, L.stream_handle()), self.ctx.handle,
-                "pol_multiphase_step")
-        self._cur ^= 1
-        self._steps += 1
+        self.update_phase_field_cahn_hilliard()
 
     # The reference's step runs these four in a row (multiphase_3d.py:405-414);
     # on the device they are one pass, run by the first of them.
