@@ -681,7 +681,7 @@ def main():
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: the multi-GPU (z-slab halo exchange) path is not part of this build",
+        print("bench.py runs on one GPU only: the multi-GPU (z-slab halo exchange) path is not implemented in this package",
               file=sys.stderr)
         sys.exit(2)
 

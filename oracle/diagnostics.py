@@ -1,7 +1,7 @@
 """numpy restatement of the reference's field diagnostics.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ and bench.py's cpu_baseline leg
-as the checker of the device diagnostics (diagnostics.hip, diagnostics.py)
+as the reference of comparison for the device diagnostics (diagnostics.hip, diagnostics.py)
 -- never by the product.
 
 Arrays follow the reference's field order: scalars [nx, ny, nz], vectors

@@ -1,6 +1,6 @@
 """numpy restatement of the reference's body-force producers.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference of comparison for
 csrc/forcing.hip -- never by the product.
 
 Follows src/physics/precise_pouring.py:83-193 (pour position, region

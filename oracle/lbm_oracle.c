@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library and its
  * Python host layer) may load or call this; it is used by tests/, by
- * __graft_entry__.smoke() as the checker, and by bench.py's cpu_baseline
+ * __graft_entry__.smoke() as the reference of comparison, and by bench.py's cpu_baseline
  * leg.  Compiled twice from this one file: REAL=double (suffix _f64) and
  * REAL=float (suffix _f32); OpenMP over z-planes.
  *

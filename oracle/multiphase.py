@@ -1,7 +1,7 @@
 """numpy restatement of the reference's multiphase phase-field producer.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ and bench.py's cpu_baseline leg
-as the checker of csrc/multiphase.hip -- never by the product.
+as the reference of comparison for csrc/multiphase.hip -- never by the product.
 
 Follows src/core/multiphase_3d.py as the class resolves at run time (the
 later of each duplicated method wins: compute_surface_tension_force at

@@ -1,7 +1,7 @@
 """ctypes front-end of the C oracle (oracle/lbm_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg -- never by the product.
+the reference of comparison and by bench.py's cpu_baseline leg -- never by the product.
 
 Arrays follow the device layout: populations [19, nz, ny, nx], scalar fields
 [nz, ny, nx], vector fields [3, nz, ny, nx], particle vectors [3, P].

@@ -1,7 +1,7 @@
 """numpy restatement of the reference's thermal D3Q7 solver.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ and bench.py's cpu_baseline leg
-as the checker of csrc/thermal.hip -- never by the product.
+as the reference of comparison for csrc/thermal.hip -- never by the product.
 
 Follows src/physics/thermal_lbm.py in the reference's array order (fields
 [nx, ny, nz], populations [nx, ny, nz, 7], q = rest, +x, -x, +y, -y, +z, -z)

@@ -424,7 +424,7 @@ class ComputeBackendFactory:
     def _create_backend_with_validation(self, backend_type: str, validate: bool = True) -> ComputeBackend:
         if backend_type in _REFERENCE_ONLY:
             raise BackendInitializationError("%s backend is a Taichi backend of the reference and is not part of "
-                                             "this build" % backend_type, backend_type, "IMPORT_FAILED")
+                                             "this package" % backend_type, backend_type, "IMPORT_FAILED")
         ctor = _REGISTRY.get(backend_type)
         if ctor is None:
             raise BackendInitializationError("unknown backend type: %s" % backend_type, backend_type,

@@ -107,7 +107,7 @@ class SolverConfig:
         return d
 
     def oracle_dict(self, use_flags: bool) -> dict:
-        """Keyword form of the same descriptor (for checkers)."""
+        """Keyword form of the same descriptor (for tests)."""
         return dict(nx=self.nx, ny=self.ny, nz=self.nz, tau=self.tau, options=self.options(use_flags),
                     cs_smag=self.cs_smag, nu_sgs_max=self.nu_sgs_max,
                     gravity=tuple(self.gravity or (0.0, 0.0, 0.0)), darcy_K=self.darcy_K, forch_CF=self.forch_CF,

@@ -36,7 +36,7 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def hip():
-    """Loaded HIP library + a visible device, else the test is skipped... never on a GPU box."""
+    """Loaded HIP library + a visible device, else the test is skipped."""
     import torch
 
     if not torch.cuda.is_available():
