@@ -139,3 +139,27 @@ def test_full_size_coupled_run_stays_physical(hip):
     assert (kind == 0).all()
     assert not wl.geometry.porous[ci[2], ci[1], ci[0]].any()
     assert np.isfinite(pos).all()
+
+
+def test_coupled_step_without_particles_is_the_plain_step(hip):
+    """An empty particle set (count 0): the coupled loop launches no particle
+    work, the drag field stays zero, and the fluid ends bit for bit where the
+    plain step takes it."""
+    print("case: coupled step with 0 particles, 32^3 cup, 20 steps", flush=True)
+    wl = WL.v60_config(32, "f32", particle_field=True)
+    a = HipContext(wl.config, wl.geometry.flags)
+    b = HipContext(wl.config, wl.geometry.flags)
+    a.init_equilibrium()
+    b.init_equilibrium()
+    none = np.zeros(0, np.float32)
+    ps = HipCoffeeParticleSystem(a, np.zeros((3, 0), np.float32), none, none, gravity=particle_gravity(1e-5, 1.2))
+    ps.prime()
+    ps.coupled_step(nsteps=20)
+    b.step(20, L.POL_WANT_MACRO)
+    torch.cuda.synchronize()
+    assert a.current == b.current
+    assert torch.equal(a.f_current.view(torch.int32), b.f_current.view(torch.int32))
+    assert torch.equal(a.rho.view(torch.int32), b.rho.view(torch.int32))
+    assert torch.equal(a.u.view(torch.int32), b.u.view(torch.int32))
+    assert float(a.pdrag.abs().max()) == 0.0
+    assert ps.get_coupling_diagnostics()["active_particles"] == 0
