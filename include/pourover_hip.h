@@ -531,7 +531,9 @@ int pol_density_drive(pol_ctx *ctx, const float *target, double rate, int half, 
 /* compute_pressure_gradient (pressure_gradient_drive.py:124-173) into
  * pressure_force (float32 [3][cells], fluid cells) and, mode 1, body_force +=
  * pressure_force (:186-190) or, mode 2, body_force += 0.5 pressure_force
- * (:276-279); mode 0 stores only. */
+ * (:276-279); mode 0 stores only.  pressure_force must hold zero at solid
+ * cells, as the reference's field does (it starts at zero and only this pass
+ * writes it); the pass may store that zero again. */
 int pol_pressure_force(pol_ctx *ctx, float *pressure_force, int mode, double max_force, double cs2, void *stream);
 /* update_dynamic_resistance (filter_paper.py:699-746) on the paper cells
  * listed in cells (ncells cell indices, the cells with filter_zone == 1);

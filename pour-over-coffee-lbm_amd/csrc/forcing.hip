@@ -97,13 +97,34 @@ __global__ void __launch_bounds__(256) k_density_drive(const ForcingArgs f, cons
 
 // pressure_force = -grad(rho) cs2 / rho (one-sided on the faces), limited to
 // max_force, on fluid cells; body_force += scale pressure_force
-// (pressure_gradient_drive.py:124-173, 186-190, 276-279)
+// (pressure_gradient_drive.py:124-173, 186-190, 276-279).  Solid cells are
+// never formed and hold zero (the field starts at zero and only this pass
+// writes it); a solid cell that shares its wave with fluid stores that zero
+// again, so that the wave's lines of the field are written whole rather than
+// read, merged and written back.
 template <typename R>
 __global__ void __launch_bounds__(256) k_pressure_force(const ForcingArgs f, float* __restrict__ pf, float max_force,
                                                         float cs2, int mode) {
 #pragma clang fp contract(off)
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
-  if (c >= f.N || fc_solid(f, c)) return;
+  if (c >= f.N) return;
+  const size_t N = f.N;
+  const bool solid = fc_solid(f, c);
+  if (__ballot(!solid) == 0) return;  // an all-solid wave touches nothing
+  if (solid) {
+    pf[c] = 0.0f;
+    pf[N + c] = 0.0f;
+    pf[2 * N + c] = 0.0f;
+    return;
+  }
+  R* __restrict__ bf = (R*)f.force;
+  // the body force is read up front, beside the density stencil
+  R b0 = R(0), b1 = R(0), b2 = R(0);
+  if (mode != 0) {
+    b0 = bf[c];
+    b1 = bf[N + c];
+    b2 = bf[2 * N + c];
+  }
   const uint32_t yz = fdiv(c, f.dnx);
   const uint32_t x = c - yz * f.nx;
   const uint32_t z = fdiv(yz, f.dny);
@@ -135,20 +156,18 @@ __global__ void __launch_bounds__(256) k_pressure_force(const ForcingArgs f, flo
       f2 = f2 * s;
     }
   }
-  const size_t N = f.N;
   pf[c] = f0;
   pf[N + c] = f1;
   pf[2 * N + c] = f2;
   if (mode == 0) return;
-  R* __restrict__ bf = (R*)f.force;
   if (mode == 1) {
-    bf[c] = bf[c] + (R)f0;
-    bf[N + c] = bf[N + c] + (R)f1;
-    bf[2 * N + c] = bf[2 * N + c] + (R)f2;
+    bf[c] = b0 + (R)f0;
+    bf[N + c] = b1 + (R)f1;
+    bf[2 * N + c] = b2 + (R)f2;
   } else {
-    bf[c] = bf[c] + (R)(0.5f * f0);
-    bf[N + c] = bf[N + c] + (R)(0.5f * f1);
-    bf[2 * N + c] = bf[2 * N + c] + (R)(0.5f * f2);
+    bf[c] = b0 + (R)(0.5f * f0);
+    bf[N + c] = b1 + (R)(0.5f * f1);
+    bf[2 * N + c] = b2 + (R)(0.5f * f2);
   }
 }
 
