@@ -582,8 +582,9 @@ def bench_forcing(n=224, reps=100, warmup=10):
 def bench_main_loop(n=224, reps=50, warmup=12):
     """The reference's whole main-loop step (main.py:769-839, from step 11
     on) on the 224^3 V60 domain, FP32, every part on the device: clear the
-    body force, pour force and phase change, pressure drive (force mode),
-    pre-collision surface tension, the stream-collide step writing rho / u,
+    body force, pour force and phase change, pressure drive (force mode)
+    with the pre-collision surface tension in the same pass (from step 11
+    on), the stream-collide step writing rho / u,
     the filter blockage update and the phase step.  HIP events around
     `reps` consecutive steps."""
     import torch
@@ -612,9 +613,10 @@ def bench_main_loop(n=224, reps=50, warmup=12):
         ctx.force.zero_()
         pour.apply_pouring_force(1.0)
         pour.apply_gradual_phase_change(mp._phi[mp._cur], None, 1.0)
-        drive.apply(count[0])
-        if count[0] > 10:
-            mp.accumulate_surface_tension_pre_collision()
+        if count[0] > 10:           # pressure drive and surface tension in one pass
+            drive.apply_with_surface_tension(mp, count[0])
+        else:
+            drive.apply(count[0])
         ctx.step(1, L.POL_WANT_MACRO)
         paper.update_dynamic_resistance()
         mp.step(count[0], precollision_applied=True)

@@ -533,7 +533,13 @@ int pol_density_drive(pol_ctx *ctx, const float *target, double rate, int half, 
  * pressure_force (:186-190) or, mode 2, body_force += 0.5 pressure_force
  * (:276-279); mode 0 stores only.  pressure_force must hold zero at solid
  * cells, as the reference's field does (it starts at zero and only this pass
- * writes it); the pass may store that zero again. */
+ * writes it); the pass may store that zero again.
+ * mode | POL_PRESSURE_SURFACE_TENSION (modes 1 and 2, a bound phase field):
+ * the pre-collision surface tension of pol_multiphase_interface(POL_MP_APPLY)
+ * (multiphase_3d.py:403-421) follows in the same pass on the sum just formed
+ * -- the result of the two calls in a row, bit for bit, with one read and one
+ * write of the body force instead of two. */
+#define POL_PRESSURE_SURFACE_TENSION 0x4
 int pol_pressure_force(pol_ctx *ctx, float *pressure_force, int mode, double max_force, double cs2, void *stream);
 /* update_dynamic_resistance (filter_paper.py:699-746) on the paper cells
  * listed in cells (ncells cell indices, the cells with filter_zone == 1);

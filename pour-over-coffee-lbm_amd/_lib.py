@@ -97,6 +97,7 @@ STAT_SLOTS = ("cells", "sum_drho", "sum_drho2", "max_abs_drho", "mom_x", "mom_y"
 POL_THERMAL_CONVECTION = 0x1
 POL_THERMAL_RESET_SOURCE = 0x2
 POL_MP_APPLY = 0x1
+POL_PRESSURE_SURFACE_TENSION = 0x4
 
 STORAGE_CODES = {"f32": POL_F32, "f16": POL_F16, "bf16": POL_BF16, "f64": POL_F64}
 

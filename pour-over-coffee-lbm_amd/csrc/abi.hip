@@ -778,15 +778,26 @@ int pol_density_drive(pol_ctx* c, const float* target, double rate, int half, do
 }
 
 int pol_pressure_force(pol_ctx* c, float* pressure_force, int mode, double max_force, double cs2, void* stream) {
-  if (!c || !pressure_force || mode < 0 || mode > 2) return POL_ERR_INVALID;
+  if (!c || !pressure_force) return POL_ERR_INVALID;
+  const bool st = (mode & POL_PRESSURE_SURFACE_TENSION) != 0;
+  mode &= ~POL_PRESSURE_SURFACE_TENSION;
+  if (mode < 0 || mode > 2) return fail(c, POL_ERR_INVALID, "pol_pressure_force: mode must be 0, 1 or 2");
   if (!c->bound || !c->b.rho) return fail(c, POL_ERR_STATE, "pol_pressure_force: buffers not bound");
   const ForcingArgs f = forcing_args(c);
   if (mode && !f.force)
     return fail(c, POL_ERR_STATE, "pol_pressure_force: modes 1 and 2 need the context's force field");
+  if (st && mode == 0)
+    return fail(c, POL_ERR_INVALID, "pol_pressure_force: the surface tension follows modes 1 and 2 only");
+  if (st && !c->mp) return fail(c, POL_ERR_STATE, "pol_pressure_force: surface tension needs a bound phase field");
+  MultiphaseArgs m{};
+  if (st) {
+    m = mp_args(c);
+    m.apply = 1u;
+  }
   hipError_t e = hipSetDevice(c->d.device);
   if (e == hipSuccess)
-    e = launch_pressure_force(c->d.storage == POL_F64, f, pressure_force, (float)max_force, (float)cs2, mode,
-                              (hipStream_t)stream);
+    e = launch_pressure_force(c->d.storage == POL_F64, f, st ? &m : nullptr, pressure_force, (float)max_force,
+                              (float)cs2, mode, (hipStream_t)stream);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_pressure_force");
 }
 

@@ -137,8 +137,10 @@ struct FilterParticleArgs {
 hipError_t launch_pour(bool dbl, const ForcingArgs& f, const PourArgs& p, float* phi, hipStream_t s);
 hipError_t launch_density_drive(bool dbl, const ForcingArgs& f, const float* target, float rate, bool half,
                                 float max_adj, float lo, float hi, hipStream_t s);
-hipError_t launch_pressure_force(bool dbl, const ForcingArgs& f, float* pf, float max_force, float cs2, int mode,
-                                 hipStream_t s);
+// st: the bound phase field whose pre-collision surface tension follows in
+// the same pass (modes 1 and 2), or null
+hipError_t launch_pressure_force(bool dbl, const ForcingArgs& f, const MultiphaseArgs* st, float* pf, float max_force,
+                                 float cs2, int mode, hipStream_t s);
 hipError_t launch_filter_blockage(const uint32_t* cells, uint32_t ncells, float* acc, float* blockage, hipStream_t s);
 hipError_t launch_filter_block(const FilterParticleArgs& a, hipStream_t s);
 

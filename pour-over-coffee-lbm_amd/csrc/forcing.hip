@@ -18,6 +18,7 @@
 // kernel computes the force and adds it to the body force in one pass.
 // float32 in the reference's operation order with contraction off.
 #include "engine.h"
+#include "phase_cell.h"
 
 namespace pol {
 
@@ -102,9 +103,14 @@ __global__ void __launch_bounds__(256) k_density_drive(const ForcingArgs f, cons
 // writes it); a solid cell that shares its wave with fluid stores that zero
 // again, so that the wave's lines of the field are written whole rather than
 // read, merged and written back.
-template <typename R>
-__global__ void __launch_bounds__(256) k_pressure_force(const ForcingArgs f, float* __restrict__ pf, float max_force,
-                                                        float cs2, int mode) {
+// ST: the pre-collision surface tension of the bound phase field follows in
+// the same pass (multiphase_3d.py:403-411 on the sum just formed): b' = b +
+// scale pf, then b' + F / rho where the separate pass would add it, so the
+// result is the two passes' bit for bit with one read and one write of the
+// body force instead of two.
+template <typename R, bool ST>
+__global__ void __launch_bounds__(256) k_pressure_force(const ForcingArgs f, const MultiphaseArgs m,
+                                                        float* __restrict__ pf, float max_force, float cs2, int mode) {
 #pragma clang fp contract(off)
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   if (c >= f.N) return;
@@ -160,15 +166,41 @@ __global__ void __launch_bounds__(256) k_pressure_force(const ForcingArgs f, flo
   pf[N + c] = f1;
   pf[2 * N + c] = f2;
   if (mode == 0) return;
+  R s0, s1, s2;
   if (mode == 1) {
-    bf[c] = b0 + (R)f0;
-    bf[N + c] = b1 + (R)f1;
-    bf[2 * N + c] = b2 + (R)f2;
+    s0 = b0 + (R)f0;
+    s1 = b1 + (R)f1;
+    s2 = b2 + (R)f2;
   } else {
-    bf[c] = b0 + (R)(0.5f * f0);
-    bf[N + c] = b1 + (R)(0.5f * f1);
-    bf[2 * N + c] = b2 + (R)(0.5f * f2);
+    s0 = b0 + (R)(0.5f * f0);
+    s1 = b1 + (R)(0.5f * f1);
+    s2 = b2 + (R)(0.5f * f2);
   }
+  if constexpr (ST) {
+    // k_mp_apply's cell update on s (multiphase_3d.py:365-381, 403-411)
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+    bool force = false;
+    if (mp_interior(m, x, y, z)) {
+      const float p = m.phi[c];
+      if (fabsf(p) < 0.9f) {
+        const CellForce cf = cell_force(m, c, x, y, z, p);
+        t0 = cf.f0;
+        t1 = cf.f1;
+        t2 = cf.f2;
+        force = true;
+      }
+    }
+    const bool negzero = (s0 == R(0) && signbit(s0)) || (s1 == R(0) && signbit(s1)) || (s2 == R(0) && signbit(s2));
+    const R rc = rho[c];
+    if ((force || negzero) && rc > R(1e-10f)) {
+      s0 = s0 + (R)t0 / rc;
+      s1 = s1 + (R)t1 / rc;
+      s2 = s2 + (R)t2 / rc;
+    }
+  }
+  bf[c] = s0;
+  bf[N + c] = s1;
+  bf[2 * N + c] = s2;
 }
 
 // filter_paper.py:699-746 on the listed paper cells
@@ -250,11 +282,17 @@ hipError_t launch_density_drive(bool dbl, const ForcingArgs& f, const float* tar
   return hipGetLastError();
 }
 
-hipError_t launch_pressure_force(bool dbl, const ForcingArgs& f, float* pf, float max_force, float cs2, int mode,
-                                 hipStream_t s) {
+hipError_t launch_pressure_force(bool dbl, const ForcingArgs& f, const MultiphaseArgs* st, float* pf, float max_force,
+                                 float cs2, int mode, hipStream_t s) {
   const dim3 grid((f.N + 255) / 256);
-  if (dbl) hipLaunchKernelGGL(k_pressure_force<double>, grid, dim3(256), 0, s, f, pf, max_force, cs2, mode);
-  else hipLaunchKernelGGL(k_pressure_force<float>, grid, dim3(256), 0, s, f, pf, max_force, cs2, mode);
+  const MultiphaseArgs m = st ? *st : MultiphaseArgs{};
+  if (st) {
+    if (dbl) hipLaunchKernelGGL((k_pressure_force<double, true>), grid, dim3(256), 0, s, f, m, pf, max_force, cs2, mode);
+    else hipLaunchKernelGGL((k_pressure_force<float, true>), grid, dim3(256), 0, s, f, m, pf, max_force, cs2, mode);
+  } else {
+    if (dbl) hipLaunchKernelGGL((k_pressure_force<double, false>), grid, dim3(256), 0, s, f, m, pf, max_force, cs2, mode);
+    else hipLaunchKernelGGL((k_pressure_force<float, false>), grid, dim3(256), 0, s, f, m, pf, max_force, cs2, mode);
+  }
   return hipGetLastError();
 }
 

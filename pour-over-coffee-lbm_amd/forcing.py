@@ -250,6 +250,20 @@ class PressureGradientDrive:
         if self.mixed_drive_active == 1:
             self.compute_pressure_gradient(mode=2)
 
+    def apply_with_surface_tension(self, multiphase, step: int = 0) -> None:
+        """apply(step) followed by multiphase.accumulate_surface_tension_pre_collision()
+        (main.py:786-800).  When the drive adds its force to the body force
+        (force or mixed drive) and the phase field is bound to the same
+        context, both run as one device pass with the same result bit for
+        bit; otherwise the two calls run in a row."""
+        mode = 1 if self.force_drive_active == 1 else (2 if self.mixed_drive_active == 1 else 0)
+        same_ctx = getattr(multiphase, "ctx", None) is self.ctx
+        if mode == 0 or not same_ctx or self.density_drive_active == 1:
+            self.apply(step)
+            multiphase.accumulate_surface_tension_pre_collision()
+            return
+        self.compute_pressure_gradient(mode=mode | L.POL_PRESSURE_SURFACE_TENSION)
+
     def get_status(self) -> dict:
         return {"density_drive": bool(self.density_drive_active), "force_drive": bool(self.force_drive_active),
                 "mixed_drive": bool(self.mixed_drive_active)}

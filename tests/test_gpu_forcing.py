@@ -160,3 +160,42 @@ def test_filter_dynamics():
     _same(tv.cpu().numpy(), wv, "particle velocities")
     _same(filt._acc.cpu().numpy(), wa, "accumulated particles after blocking")
     assert (wv[2] != vel[2]).sum() > 5
+
+
+@pytest.mark.parametrize("mode", ["force", "mixed"])
+def test_pressure_drive_with_surface_tension_in_one_pass(mode):
+    """PressureGradientDrive.apply_with_surface_tension (one pass) against the
+    pressure drive followed by the separate surface-tension pass: pressure
+    force and body force bit for bit, on a droplet interface with random
+    density (one cell at rho <= 1e-10) and planted -0.0 components."""
+    from pourover_amd import multiphase as MP
+
+    print("case: pressure drive + surface tension in one pass, %s" % mode, flush=True)
+    fluid = _solver()
+    rng = np.random.default_rng(9)
+    rho = rng.uniform(0.9, 1.1, (NZ, NY, NX)).astype(F)
+    rho[12, 10, 11] = F(1e-11)
+    bf = rng.normal(0, 1e-3, (3, NZ, NY, NX)).astype(F)
+    bf[:, 5, 5, 5] = F(-0.0)
+    bf[2, 12, 10, 12] = F(-0.0)
+    z, y, x = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    r = np.sqrt((x - 11.3) ** 2 + (y - 10.1) ** 2 + (z - 11.7) ** 2)
+    phi = np.tanh((6.0 - r) / 2.0).astype(F)
+    mp = MP.MultiphaseFlow3D(fluid)
+    mp._phi[mp._cur].copy_(_dev(phi))
+    fluid.ctx.rho.copy_(_dev(rho))
+    fluid._macro_valid = True
+    drive = FC.PressureGradientDrive(fluid)
+    getattr(drive, "activate_%s_drive" % mode)(True)
+    fluid.ctx.force.copy_(_dev(bf))
+    drive.apply(3)
+    mp.accumulate_surface_tension_pre_collision()
+    torch.cuda.synchronize()
+    pf_two, bf_two = drive._pforce.cpu().numpy(), fluid.ctx.force.cpu().numpy()
+    assert np.abs(bf_two - bf).max() > 1e-4            # the interface carries a force
+    drive._pforce.zero_()
+    fluid.ctx.force.copy_(_dev(bf))
+    drive.apply_with_surface_tension(mp, 3)
+    torch.cuda.synchronize()
+    _same(drive._pforce.cpu().numpy(), pf_two, "pressure force")
+    _same(fluid.ctx.force.cpu().numpy(), bf_two, "body force")

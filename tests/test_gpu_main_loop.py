@@ -43,9 +43,12 @@ def _refv(t):  # device [3, z, y, x] -> reference [x, y, z, 3]
     return np.ascontiguousarray(t.detach().cpu().numpy().transpose(3, 2, 1, 0))
 
 
-def test_main_loop_step_matches_oracles():
+@pytest.mark.parametrize("fused", [False, True])
+def test_main_loop_step_matches_oracles(fused):
+    """fused: the pressure drive and the surface tension as one pass
+    (PressureGradientDrive.apply_with_surface_tension) from step 11 on."""
     n = 40
-    print("case: main-loop step V60 %d^3, 14 steps" % n, flush=True)
+    print("case: main-loop step V60 %d^3, 14 steps, fused=%s" % (n, fused), flush=True)
     wl = WL.v60_config(n, "f32", force_field=True)
     ctx = HipContext(wl.config, wl.geometry.flags)
     ctx.init_equilibrium()
@@ -65,9 +68,12 @@ def test_main_loop_step_matches_oracles():
         ctx.force.zero_()
         pour.apply_pouring_force(1.0)
         pour.apply_gradual_phase_change(mp._phi[mp._cur], None, 1.0)
-        drive.apply(step)
-        if step > 10:
-            mp.accumulate_surface_tension_pre_collision()
+        if fused and step > 10:
+            drive.apply_with_surface_tension(mp, step)
+        else:
+            drive.apply(step)
+            if step > 10:
+                mp.accumulate_surface_tension_pre_collision()
         bf_dev = ctx.force.cpu().numpy()
         ctx.step(1, L.POL_WANT_MACRO)
         paper.update_dynamic_resistance()
