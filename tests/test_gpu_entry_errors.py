@@ -105,6 +105,12 @@ def test_forcing_entry_errors():
     st, msg = _status(ctx, lib.pol_pressure_force, pf.data_ptr(), 1, 0.12, 1 / 3, s)
     assert st == L.POL_ERR_STATE and "force" in msg
     assert lib.pol_pressure_force(ctx.handle, pf.data_ptr(), 0, 0.12, 1 / 3, s) == L.POL_OK  # store only
+    st, msg = _status(ctx, lib.pol_pressure_force, pf.data_ptr(), L.POL_PRESSURE_SURFACE_TENSION, 0.12, 1 / 3, s)
+    assert st == L.POL_ERR_INVALID and "modes 1 and 2" in msg     # nothing to add the tension to
+    assert lib.pol_pressure_force(ctx.handle, pf.data_ptr(), 8, 0.12, 1 / 3, s) == L.POL_ERR_INVALID
+    fctx = _ctx(force=True)
+    st, msg = _status(fctx, lib.pol_pressure_force, pf.data_ptr(), 1 | L.POL_PRESSURE_SURFACE_TENSION, 0.12, 1 / 3, s)
+    assert st == L.POL_ERR_STATE and "phase field" in msg          # no phase field bound
     cells = torch.zeros(4, dtype=torch.int32, device="cuda")
     st, msg = _status(ctx, lib.pol_filter_blockage, cells.data_ptr(), NX * NY * NZ + 1, phi.data_ptr(),
                       phi.data_ptr(), s)
