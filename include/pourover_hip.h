@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define POL_ABI_VERSION 1
+#define POL_ABI_VERSION 2
 #define POL_Q 19
 #define POL_NRED 12 /* length of the reduction vector, see POL_RED_* */
 
@@ -185,8 +185,10 @@ const char *pol_last_error(const pol_ctx *ctx);
  * contract f/f_new/rho/u/solid (src/core/backends/cpu_backend.py:207-212). */
 int pol_bind(pol_ctx *ctx, const pol_buffers *bufs);
 /* Replace the physics scalars (tau, cs_smag, nu_sgs_max, gravity, darcy_K,
- * forch_CF, inlet/outlet values) of a context; nx/ny/nz, storage, options and
- * device must equal the context's.  Takes effect for launches enqueued
+ * forch_CF, inlet/outlet values) of a context; nx/ny/nz, storage and device
+ * must equal the context's, and so must the options except POL_LES, which
+ * may be switched (LBMSolverProtocol.enable_les_turbulence,
+ * src/core/lbm_protocol.py:168).  Takes effect for launches enqueued
  * afterwards.  Replaces setting config module constants between runs
  * (config/core.py:57-93, config/physics.py:255-273). */
 int pol_set_params(pol_ctx *ctx, const pol_desc *desc);
@@ -242,7 +244,22 @@ typedef struct pol_particle_desc {
   double dt;             /* particle time step (lattice units) */
   double max_disp;       /* displacement clamp per step */
   double wall_damping;   /* velocity factor when a move is rejected */
+  int32_t reaction;      /* POL_REACTION_IMPLICIT or POL_REACTION_EXPLICIT */
+  int32_t reserved;      /* 0 */
 } pol_particle_desc;
+
+/* How the particles act back on the fluid (pol_particle_desc.reaction).
+ *   IMPLICIT: the coupling spreads (w beta v_p, w beta) into the lattice's
+ *             particle-drag field (pol_buffers.particle_drag, POL_PARTICLE_FIELD)
+ *             and the collision applies sum w beta (v_p - u) with the cell
+ *             velocity u implicit (stable for any drag coefficient).
+ *   EXPLICIT: the reference's form: the coupling spreads -drag_new into a
+ *             [3][cells] reaction field (pol_particle_buffers.reaction_field,
+ *             coffee_particles.py:1146-1149,1049-1086), which
+ *             pol_add_reaction_forces adds to the body force on the fluid
+ *             cells (legacy/lbm_solver.py:1478-1483). */
+#define POL_REACTION_IMPLICIT 0
+#define POL_REACTION_EXPLICIT 1
 
 /* All vectors SoA [3][count], in the real type of the context (double for
  * POL_F64 storage, float otherwise); active is int32. */
@@ -252,23 +269,47 @@ typedef struct pol_particle_buffers {
   void *drag_new, *drag_old, *drag; /* [3][count] */
   void *u_fluid;  /* [3][count] interpolated fluid velocity (diagnostic) */
   void *re_p, *cd; /* [count] diagnostics */
-  void *drag_field; /* [4][cells] particle-drag field (= pol_buffers.particle_drag) */
+  void *drag_field; /* [4][cells] particle-drag field (= pol_buffers.particle_drag); IMPLICIT */
   const void *u_field; /* [3][cells] fluid velocity field read by the gather */
+  void *reaction_field; /* [3][cells] reaction field (real type); EXPLICIT, else NULL */
 } pol_particle_buffers;
 
 /* Attach a particle system.  Replaces CoffeeParticleSystem field set-up
  * (coffee_particles.py:30-64). */
 int pol_particles_bind(pol_ctx *ctx, const pol_particle_desc *d, const pol_particle_buffers *b);
-/* Interpolate, drag, under-relax and spread the drag coefficient
- * (w beta v_p, w beta) into the particle-drag field.  Replaces
+/* Interpolate, drag, under-relax and spread the reaction: IMPLICIT adds
+ * (w beta v_p, w beta) to the particle-drag field, EXPLICIT clears the
+ * reaction field and adds w (-drag_new) to it.  A second call before
+ * pol_particles_advance recomputes (IMPLICIT clears its own earlier
+ * footprint first), so the field never holds two couplings.  Replaces
  * compute_two_way_coupling_forces + apply_under_relaxation
  * (coffee_particles.py:1107-1154,1200-1212). */
 int pol_particles_couple(pol_ctx *ctx, void *stream);
+/* compute_two_way_coupling_forces(fluid_u) (coffee_particles.py:1107-1154)
+ * alone: interpolate u_field (NULL: the bound u_field), drag_new, Reynolds
+ * number and C_D per particle, and the reaction as in pol_particles_couple,
+ * without the under-relaxation. */
+int pol_particles_compute_coupling(pol_ctx *ctx, const void *u_field, void *stream);
+/* apply_under_relaxation(alpha) (coffee_particles.py:1200-1212): on active
+ * particles drag = alpha drag_new + (1 - alpha) drag_old; drag_old = drag.
+ * alpha in [0, 1]. */
+int pol_particles_relax(pol_ctx *ctx, double alpha, void *stream);
+/* clear_reaction_forces (coffee_particles.py:1101-1105): EXPLICIT zeroes the
+ * reaction field; IMPLICIT zeroes the drag-field footprint of the last
+ * coupling if it is still there. */
+int pol_particles_clear_reaction(pol_ctx *ctx, void *stream);
+/* add_particle_reaction_forces (legacy/lbm_solver.py:1478-1483): the body
+ * force (pol_buffers.force, POL_FORCE_FIELD) takes reaction_field ([3][cells],
+ * real type; NULL: the bound particle system's reaction field) on every
+ * cell whose kind is not solid. */
+int pol_add_reaction_forces(pol_ctx *ctx, const void *reaction_field, void *stream);
 /* Clear this step's particle-drag footprint and advance the particles.
  * Replaces clear_reaction_forces + update_particle_physics
  * (coffee_particles.py:1101-1105,641-720). */
 int pol_particles_advance(pol_ctx *ctx, void *stream);
-/* nsteps two-way coupled steps, each: pol_particles_couple, one fused
+/* nsteps two-way coupled steps, each: pol_particles_couple, (EXPLICIT: the
+ * body force is cleared and takes the reaction field, as
+ * step_with_two_way_coupling clears and refills it), one fused
  * stream-collide step writing rho/u (the velocity the next coupling
  * interpolates), pol_particles_advance.  The last step also fills red_out
  * with `what & POL_WANT_REDUCE` (synchronous then).  Replaces the per-step

@@ -26,8 +26,10 @@
  *                          with the hard-coded opposite table (:168-190)
  *   oracle_particle_index  coffee_particles.py:1159-1174 in float32
  *   oracle_particles_*     coffee_particles.py:1088-1154,1200-1212 (drag,
- *                          scatter, under-relaxation) and the build's
- *                          particle advance (replacing :641-720)
+ *                          scatter into the implicit drag field or the
+ *                          reference's explicit reaction field
+ *                          :1146-1149,1049-1086, under-relaxation) and the
+ *                          build's particle advance (replacing :641-720)
  *
  * Parity pins (tests/): closed forms (lattice identities, Taylor-Green decay,
  * Guo momentum law, porous terminal velocity, LES uniform shear, settling
@@ -384,14 +386,19 @@ typedef struct {
   real rho_fluid, nu_fluid, drag_max_per_mass, alpha, g[3], dt, max_disp, wall_damping;
 } oracle_particle_params;
 
-/* couple: gather u, drag = beta (u_f - v), accumulate the [4][cells] drag field
- * (w beta v, w beta) in `force`, relax.  The reference scatters -drag
- * explicitly (coffee_particles.py:1148-1149); the field form lets the
- * collision treat the reaction implicitly. */
-void SFX(oracle_particles_couple)(int nx, int ny, int nz, const oracle_particle_params *pp, const real *pos,
-                                  const real *vel, const real *radius, const real *mass, const int32_t *active,
-                                  const real *u_field, real *force, real *u_fluid, real *drag_new, real *drag_old,
-                                  real *drag, real *re_out, real *cd_out) {
+/* Coupling pass of one particle set (coffee_particles.py:1107-1154).
+ * Gather u at the particle, drag = beta (u_f - v) with the Schiller-Naumann
+ * C_D, then the reaction:
+ *   explicit = 0: accumulate the [4][cells] drag field (w beta v, w beta) in
+ *                 `field` (the build's implicit form);
+ *   explicit = 1: accumulate w (-drag_new) in the [3][cells] reaction field
+ *                 `field`, as distribute_force_to_grid does
+ *                 (coffee_particles.py:1146-1149,1049-1086).
+ * relax = 1 also applies the under-relaxation (coffee_particles.py:1200-1212). */
+static void couple_pass(int nx, int ny, int nz, const oracle_particle_params *pp, const real *pos, const real *vel,
+                        const real *radius, const real *mass, const int32_t *active, const real *u_field, real *field,
+                        real *u_fluid, real *drag_new, real *drag_old, real *drag, real *re_out, real *cd_out,
+                        int explicit_reaction, int relax) {
   const size_t N = (size_t)nx * ny * nz;
   const int P = pp->count;
   const size_t nxy = (size_t)nx * ny;
@@ -442,19 +449,60 @@ void SFX(oracle_particles_couple)(int nx, int ny, int nz, const oracle_particle_
       if (dm > cap) dm = cap;
       const real beta = dm / mag;
       for (int d = 0; d < 3; ++d) dn[d] = beta * rel[d];
-      /* drag field: w beta v (3 components) and w beta */
-      const real comp[4] = {beta * vel[p], beta * vel[P + p], beta * vel[2 * P + p], beta};
-      for (int d = 0; d < 4; ++d)
-        for (int k = 0; k < 8; ++k) force[(size_t)d * N + cs[k]] += ws[k] * comp[d];
+      if (explicit_reaction) {
+        for (int d = 0; d < 3; ++d)
+          for (int k = 0; k < 8; ++k) field[(size_t)d * N + cs[k]] += ws[k] * (-dn[d]);
+      } else {
+        /* drag field: w beta v (3 components) and w beta */
+        const real comp[4] = {beta * vel[p], beta * vel[P + p], beta * vel[2 * P + p], beta};
+        for (int d = 0; d < 4; ++d)
+          for (int k = 0; k < 8; ++k) field[(size_t)d * N + cs[k]] += ws[k] * comp[d];
+      }
     } else {
       re_out[p] = 0;
       cd_out[p] = 0;
     }
     for (int d = 0; d < 3; ++d) {
       drag_new[(size_t)d * P + p] = dn[d];
-      real rl = pp->alpha * dn[d] + (1 - pp->alpha) * drag_old[(size_t)d * P + p];
-      drag[(size_t)d * P + p] = rl;
-      drag_old[(size_t)d * P + p] = rl;
+      if (relax) {
+        real rl = pp->alpha * dn[d] + (1 - pp->alpha) * drag_old[(size_t)d * P + p];
+        drag[(size_t)d * P + p] = rl;
+        drag_old[(size_t)d * P + p] = rl;
+      }
+    }
+  }
+}
+
+/* couple: the implicit drag field (w beta v, w beta) in `force`, relaxed. */
+void SFX(oracle_particles_couple)(int nx, int ny, int nz, const oracle_particle_params *pp, const real *pos,
+                                  const real *vel, const real *radius, const real *mass, const int32_t *active,
+                                  const real *u_field, real *force, real *u_fluid, real *drag_new, real *drag_old,
+                                  real *drag, real *re_out, real *cd_out) {
+  couple_pass(nx, ny, nz, pp, pos, vel, radius, mass, active, u_field, force, u_fluid, drag_new, drag_old, drag,
+              re_out, cd_out, 0, 1);
+}
+
+/* compute_two_way_coupling_forces (coffee_particles.py:1107-1154) without
+ * the relaxation: explicit_reaction selects the reference's reaction field
+ * ([3][cells], accumulated: the caller clears it as :1109-1111 does) or the
+ * implicit drag field ([4][cells]). */
+void SFX(oracle_particles_compute)(int nx, int ny, int nz, const oracle_particle_params *pp, const real *pos,
+                                   const real *vel, const real *radius, const real *mass, const int32_t *active,
+                                   const real *u_field, real *field, real *u_fluid, real *drag_new, real *re_out,
+                                   real *cd_out, int explicit_reaction) {
+  couple_pass(nx, ny, nz, pp, pos, vel, radius, mass, active, u_field, field, u_fluid, drag_new, NULL, NULL, re_out,
+              cd_out, explicit_reaction, 0);
+}
+
+/* apply_under_relaxation (coffee_particles.py:1200-1212), active particles. */
+void SFX(oracle_particles_relax)(int count, real alpha, const int32_t *active, const real *drag_new, real *drag_old,
+                                 real *drag) {
+  for (int p = 0; p < count; ++p) {
+    if (!active[p]) continue;
+    for (int d = 0; d < 3; ++d) {
+      const real rl = alpha * drag_new[(size_t)d * count + p] + (1 - alpha) * drag_old[(size_t)d * count + p];
+      drag[(size_t)d * count + p] = rl;
+      drag_old[(size_t)d * count + p] = rl;
     }
   }
 }
@@ -481,8 +529,9 @@ void SFX(oracle_particles_advance)(int nx, int ny, int nz, const oracle_particle
     const size_t c000 = (size_t)ii[2] * nxy + (size_t)ii[1] * nx + ii[0];
     const size_t cs[8] = {c000, c000 + 1, c000 + nx, c000 + nx + 1, c000 + nxy, c000 + nxy + 1, c000 + nxy + nx,
                           c000 + nxy + nx + 1};
-    for (int d = 0; d < 4; ++d)
-      for (int k = 0; k < 8; ++k) force[(size_t)d * N + cs[k]] = 0;
+    if (force)
+      for (int d = 0; d < 4; ++d)
+        for (int k = 0; k < 8; ++k) force[(size_t)d * N + cs[k]] = 0;
     real v[3], uf[3], dn[3], x[3];
     for (int d = 0; d < 3; ++d) {
       v[d] = vel[(size_t)d * P + p];

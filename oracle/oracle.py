@@ -203,6 +203,43 @@ def particles_couple(dims, pp, st: dict, u_field, force, prec="f64"):
         _p(st["drag_old"]), _p(st["drag"]), _p(st["re_p"]), _p(st["cd"]))
 
 
+def particles_compute(dims, pp, st: dict, u_field, field, explicit=True, prec="f64"):
+    """compute_two_way_coupling_forces without relaxation: field is the [3][cells]
+    reaction field (explicit, accumulated: clear it first) or the [4][cells]
+    drag field (implicit)."""
+    dt = _dtype(prec)
+    P = pp.count
+    for k in ("u_fluid", "drag_new"):
+        st.setdefault(k, np.zeros((3, P), dtype=dt))
+    for k in ("re_p", "cd"):
+        st.setdefault(k, np.zeros(P, dtype=dt))
+    nx, ny, nz = dims
+    getattr(lib(), "oracle_particles_compute" + _sfx(prec))(
+        C.c_int(nx), C.c_int(ny), C.c_int(nz), C.byref(pp), _p(st["pos"]), _p(st["vel"]), _p(st["radius"]),
+        _p(st["mass"]), _p(st["active"]), _p(u_field), _p(field), _p(st["u_fluid"]), _p(st["drag_new"]),
+        _p(st["re_p"]), _p(st["cd"]), C.c_int(1 if explicit else 0))
+
+
+def particles_relax(st: dict, alpha: float, prec="f64"):
+    """apply_under_relaxation on st's drag_new / drag_old (creates drag)."""
+    dt = _dtype(prec)
+    P = st["drag_new"].shape[1]
+    st.setdefault("drag", np.zeros((3, P), dtype=dt))
+    real = C.c_double if prec == "f64" else C.c_float
+    getattr(lib(), "oracle_particles_relax" + _sfx(prec))(
+        C.c_int(P), real(alpha), _p(st["active"]), _p(st["drag_new"]), _p(st["drag_old"]), _p(st["drag"]))
+
+
+def add_reaction_forces(force: np.ndarray, reaction: np.ndarray, flags=None) -> None:
+    """legacy/lbm_solver.py:1478-1483: force += reaction on non-solid cells (in place)."""
+    if flags is None:
+        force += reaction
+        return
+    kind = (np.asarray(flags, dtype=np.uint32) >> 24) & 0x7
+    fluid = kind != 1
+    force += np.where(fluid[None], reaction, 0.0)
+
+
 def particles_advance(dims, pp, st: dict, flags, force, prec="f64"):
     nx, ny, nz = dims
     fl = np.ascontiguousarray(flags, dtype=np.uint32) if flags is not None else None

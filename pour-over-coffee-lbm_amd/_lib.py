@@ -16,7 +16,7 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libpourover_hip.so"
 
 # ---- constants (mirror include/pourover_hip.h) ------------------------------
-POL_ABI_VERSION = 1
+POL_ABI_VERSION = 2
 POL_Q = 19
 POL_NRED = 12
 POL_CHUNK = 32  # cells per launch chunk
@@ -42,6 +42,8 @@ POL_FLAG_NO_LES = 0x10000000
 POL_RFLAG_CLAMP = 0x80000000
 POL_DRAG_TEXTBOOK = 0
 POL_DRAG_REFERENCE = 1
+POL_REACTION_IMPLICIT = 0
+POL_REACTION_EXPLICIT = 1
 POL_WANT_MACRO = 0x1
 POL_WANT_REDUCE = 0x2
 RED_SLOTS = ("mass", "mom_x", "mom_y", "mom_z", "kinetic_energy", "max_velocity", "nonfinite_cells",
@@ -128,13 +130,14 @@ class PolParticleDesc(C.Structure):
         ("rho_fluid", C.c_double), ("nu_fluid", C.c_double), ("drag_max_per_mass", C.c_double),
         ("relax_alpha", C.c_double), ("gravity", C.c_double * 3), ("dt", C.c_double),
         ("max_disp", C.c_double), ("wall_damping", C.c_double),
+        ("reaction", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
 class PolParticleBuffers(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "pos", "vel", "radius", "mass", "active", "drag_new", "drag_old", "drag", "u_fluid", "re_p", "cd",
-        "drag_field", "u_field")]
+        "drag_field", "u_field", "reaction_field")]
 
 
 class PolStatsOpts(C.Structure):
@@ -196,6 +199,10 @@ SIGNATURES = {
     "pol_reduce": (C.c_int, [_P, _P, _P]),
     "pol_particles_bind": (C.c_int, [_P, C.POINTER(PolParticleDesc), C.POINTER(PolParticleBuffers)]),
     "pol_particles_couple": (C.c_int, [_P, _P]),
+    "pol_particles_compute_coupling": (C.c_int, [_P, _P, _P]),
+    "pol_particles_relax": (C.c_int, [_P, C.c_double, _P]),
+    "pol_particles_clear_reaction": (C.c_int, [_P, _P]),
+    "pol_add_reaction_forces": (C.c_int, [_P, _P, _P]),
     "pol_particles_advance": (C.c_int, [_P, _P]),
     "pol_coupled_step": (C.c_int, [_P, C.c_int, C.c_uint32, _P, _P]),
     "pol_particle_index": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P]),

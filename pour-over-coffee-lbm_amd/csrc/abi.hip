@@ -26,6 +26,7 @@ struct pol_ctx {
   double* red_dev = nullptr;  // [POL_NRED]
   // particles
   bool particles = false;
+  bool pcoupled = false;  // the implicit drag field holds the footprint of a coupling
   ParticleArgs pa{};
   // adjoint
   bool adjoint = false;
@@ -208,8 +209,11 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
 int pol_set_params(pol_ctx* c, const pol_desc* d) {
   if (!c || !d) return POL_ERR_INVALID;
   if (d->nx != c->d.nx || d->ny != c->d.ny || d->nz != c->d.nz || d->storage != c->d.storage ||
-      d->options != c->d.options || d->device != c->d.device)
-    return fail(c, POL_ERR_INVALID, "pol_set_params: geometry, storage, options and device are fixed per context");
+      (d->options & ~POL_LES) != (c->d.options & ~POL_LES) || d->device != c->d.device)
+    return fail(c, POL_ERR_INVALID,
+                "pol_set_params: geometry, storage, options (all but POL_LES) and device are fixed per context");
+  if ((d->options & POL_LES) && (!(d->cs_smag >= 0.0) || !std::isfinite(d->cs_smag) || !(d->nu_sgs_max >= 0.0)))
+    return fail(c, POL_ERR_INVALID, "pol_set_params: LES needs finite cs_smag >= 0 and nu_sgs_max >= 0");
   if (!(d->tau > 0.5) || !std::isfinite(d->tau)) return fail(c, POL_ERR_INVALID, "pol_set_params: tau must be > 1/2");
   c->d = *d;
   return POL_OK;
@@ -371,17 +375,37 @@ int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particl
   if (!c || !d || !b) return POL_ERR_INVALID;
   if (d->count < 0) return fail(c, POL_ERR_INVALID, "pol_particles_bind: negative count");
   if (d->count > (1 << 27)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: at most 2^27 particles (8 lanes each)");
-  if (!c->bound || !(c->d.options & POL_PARTICLE_FIELD))
-    return fail(c, POL_ERR_STATE, "pol_particles_bind: bind forward buffers with POL_PARTICLE_FIELD first");
+  if (d->reaction != POL_REACTION_IMPLICIT && d->reaction != POL_REACTION_EXPLICIT)
+    return fail(c, POL_ERR_INVALID, "pol_particles_bind: unknown reaction mode");
+  const bool expl = d->reaction == POL_REACTION_EXPLICIT;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_particles_bind: bind forward buffers first");
+  if (!expl && !(c->d.options & POL_PARTICLE_FIELD))
+    return fail(c, POL_ERR_STATE, "pol_particles_bind: the implicit reaction needs POL_PARTICLE_FIELD buffers");
+  if (expl && !b->reaction_field)
+    return fail(c, POL_ERR_INVALID, "pol_particles_bind: the explicit reaction needs reaction_field");
   if (d->count > 0 && (!b->pos || !b->vel || !b->radius || !b->mass || !b->active || !b->drag_new || !b->drag_old ||
-                       !b->drag || !b->u_fluid || !b->re_p || !b->cd || !b->drag_field || !b->u_field))
+                       !b->drag || !b->u_fluid || !b->re_p || !b->cd || (!expl && !b->drag_field) || !b->u_field))
     return fail(c, POL_ERR_INVALID, "pol_particles_bind: missing particle buffer");
   if (!(d->relax_alpha >= 0.0 && d->relax_alpha <= 1.0)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: alpha outside [0,1]");
   if (!(d->dt > 0.0) || !(d->max_disp > 0.0)) return fail(c, POL_ERR_INVALID, "pol_particles_bind: dt and max_disp must be > 0");
-  ParticleArgs& p = c->pa;
-  p = ParticleArgs{};
   if (d->drag_law != POL_DRAG_TEXTBOOK && d->drag_law != POL_DRAG_REFERENCE)
     return fail(c, POL_ERR_INVALID, "pol_particles_bind: unknown drag_law");
+  // A coupling footprint still in the drag field belongs to the old binding:
+  // it stays tracked when the positions and the field are the same buffers,
+  // else it is cleared now (ordered after the work already queued).
+  bool keep_coupled = false;
+  if (c->particles && c->pcoupled) {
+    if (c->pa.pos == b->pos && c->pa.drag_field == (expl ? nullptr : b->drag_field) && c->pa.count == d->count) {
+      keep_coupled = true;
+    } else {
+      hipError_t e = hipSetDevice(c->d.device);
+      if (e == hipSuccess) e = launch_particles_clear_footprint(c->pa, nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      if (e != hipSuccess) return hip_fail(c, e, "pol_particles_bind");
+    }
+  }
+  ParticleArgs& p = c->pa;
+  p = ParticleArgs{};
   p.count = d->count;
   p.real_is_double = c->d.storage == POL_F64;
   p.drag_law = d->drag_law;
@@ -402,7 +426,8 @@ int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particl
   p.u_fluid = b->u_fluid;
   p.re_p = b->re_p;
   p.cd = b->cd;
-  p.drag_field = b->drag_field;
+  p.drag_field = expl ? nullptr : b->drag_field;
+  p.reaction = expl ? b->reaction_field : nullptr;
   p.u_field = b->u_field;
   p.rho_fluid = d->rho_fluid;
   p.nu_fluid = d->nu_fluid;
@@ -413,15 +438,80 @@ int pol_particles_bind(pol_ctx* c, const pol_particle_desc* d, const pol_particl
   p.max_disp = d->max_disp;
   p.wall_damping = d->wall_damping;
   c->particles = true;
+  c->pcoupled = keep_coupled;
   return POL_OK;
+}
+
+// One coupling pass: the implicit form first clears the footprint of a
+// coupling not yet consumed by an advance (so a repeated couple recomputes
+// instead of adding twice); the explicit form clears its reaction field, as
+// compute_two_way_coupling_forces does (coffee_particles.py:1109-1111).
+static hipError_t couple_pass(pol_ctx* c, const ParticleArgs& pa, bool relax, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (pa.reaction) {
+    e = launch_zero_real(pa.real_is_double, pa.reaction, 3 * (size_t)c->N, s);
+  } else if (c->pcoupled) {
+    e = launch_particles_clear_footprint(pa, s);
+  }
+  if (e == hipSuccess) e = launch_particles_couple(pa, relax, s);
+  if (e == hipSuccess && !pa.reaction) c->pcoupled = true;
+  return e;
 }
 
 int pol_particles_couple(pol_ctx* c, void* stream) {
   if (!c) return POL_ERR_INVALID;
   if (!c->particles) return fail(c, POL_ERR_STATE, "pol_particles_couple: no particles bound");
   hipError_t e = hipSetDevice(c->d.device);
-  if (e == hipSuccess) e = launch_particles_couple(c->pa, (hipStream_t)stream);
+  if (e == hipSuccess) e = couple_pass(c, c->pa, true, (hipStream_t)stream);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_couple");
+}
+
+int pol_particles_compute_coupling(pol_ctx* c, const void* u_field, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->particles) return fail(c, POL_ERR_STATE, "pol_particles_compute_coupling: no particles bound");
+  ParticleArgs pa = c->pa;
+  if (u_field) pa.u_field = u_field;
+  if (pa.count > 0 && !pa.u_field) return fail(c, POL_ERR_INVALID, "pol_particles_compute_coupling: no velocity field");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = couple_pass(c, pa, false, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_compute_coupling");
+}
+
+int pol_particles_relax(pol_ctx* c, double alpha, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->particles) return fail(c, POL_ERR_STATE, "pol_particles_relax: no particles bound");
+  if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(c, POL_ERR_INVALID, "pol_particles_relax: alpha outside [0,1]");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_particles_relax(c->pa, alpha, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_relax");
+}
+
+int pol_particles_clear_reaction(pol_ctx* c, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->particles) return fail(c, POL_ERR_STATE, "pol_particles_clear_reaction: no particles bound");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) {
+    if (c->pa.reaction) {
+      e = launch_zero_real(c->pa.real_is_double, c->pa.reaction, 3 * (size_t)c->N, (hipStream_t)stream);
+    } else if (c->pcoupled) {
+      e = launch_particles_clear_footprint(c->pa, (hipStream_t)stream);
+      if (e == hipSuccess) c->pcoupled = false;
+    }
+  }
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_clear_reaction");
+}
+
+int pol_add_reaction_forces(pol_ctx* c, const void* reaction_field, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_add_reaction_forces: buffers not bound");
+  if (!(c->d.options & POL_FORCE_FIELD) || !c->b.force)
+    return fail(c, POL_ERR_STATE, "pol_add_reaction_forces: the lattice has no body-force field (POL_FORCE_FIELD)");
+  const void* r = reaction_field ? reaction_field : (c->particles ? c->pa.reaction : nullptr);
+  if (!r) return fail(c, POL_ERR_INVALID, "pol_add_reaction_forces: no reaction field");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess)
+    e = launch_add_field_fluid(c->d.storage == POL_F64, c->b.force, r, c->b.cell_flags, c->N, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_add_reaction_forces");
 }
 
 int pol_particles_advance(pol_ctx* c, void* stream) {
@@ -429,6 +519,7 @@ int pol_particles_advance(pol_ctx* c, void* stream) {
   if (!c->particles) return fail(c, POL_ERR_STATE, "pol_particles_advance: no particles bound");
   hipError_t e = hipSetDevice(c->d.device);
   if (e == hipSuccess) e = launch_particles_advance(c->pa, (hipStream_t)stream);
+  if (e == hipSuccess) c->pcoupled = false;
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_particles_advance");
 }
 
@@ -437,13 +528,22 @@ int pol_coupled_step(pol_ctx* c, int nsteps, uint32_t what, double* red_out, voi
   if (!c->bound) return fail(c, POL_ERR_STATE, "pol_coupled_step: buffers not bound");
   if (!c->particles) return fail(c, POL_ERR_STATE, "pol_coupled_step: no particles bound");
   if (!c->b.rho || !c->b.u) return fail(c, POL_ERR_INVALID, "pol_coupled_step: the coupling needs the rho/u buffers");
+  const bool expl = c->pa.reaction != nullptr;
+  if (expl && !((c->d.options & POL_FORCE_FIELD) && c->b.force))
+    return fail(c, POL_ERR_STATE, "pol_coupled_step: the explicit reaction needs the body-force field (POL_FORCE_FIELD)");
   int st = check_what(c, what, red_out, "pol_coupled_step");
   if (st) return st;
   if (nsteps == 0) return POL_OK;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(c->d.device);
+  const bool dbl = c->d.storage == POL_F64;
   for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
-    e = launch_particles_couple(c->pa, s);
+    e = couple_pass(c, c->pa, true, s);
+    if (e == hipSuccess && expl) {
+      // legacy/lbm_solver.py:1488,1499: clear_body_force, then add the reaction
+      e = launch_zero_real(dbl, c->b.force, 3 * (size_t)c->N, s);
+      if (e == hipSuccess) e = launch_add_field_fluid(dbl, c->b.force, c->pa.reaction, c->b.cell_flags, c->N, s);
+    }
     if (e != hipSuccess) break;
     const uint32_t w = POL_WANT_MACRO | ((i == nsteps - 1) ? what : 0u);
     StepArgs a = make_args(c, c->b.f[c->cur], c->b.f[1 - c->cur], w);
@@ -451,6 +551,7 @@ int pol_coupled_step(pol_ctx* c, int nsteps, uint32_t what, double* red_out, voi
     if (e != hipSuccess) break;
     c->cur = 1 - c->cur;
     e = launch_particles_advance(c->pa, s);
+    if (e == hipSuccess) c->pcoupled = false;
   }
   if (e != hipSuccess) return hip_fail(c, e, "pol_coupled_step");
   if (what & POL_WANT_REDUCE) return finish_reduce(c, red_out, s, "pol_coupled_step");

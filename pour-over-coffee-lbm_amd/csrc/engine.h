@@ -182,14 +182,29 @@ struct ParticleArgs {
   void *pos, *vel, *radius, *mass;
   int32_t* active;
   void *drag_new, *drag_old, *drag, *u_fluid, *re_p, *cd;
-  void* drag_field;  // [4][N]
+  void* drag_field;  // [4][N] implicit drag field, or null (explicit reaction)
+  void* reaction;    // [3][N] explicit reaction field, or null (implicit drag field)
   const void* u_field;
   double rho_fluid, nu_fluid, drag_max_per_mass, alpha;
   double g[3];
   double dt, max_disp, wall_damping;
 };
-hipError_t launch_particles_couple(const ParticleArgs& a, hipStream_t s);
+// Coupling pass.  relax: under-relax the drag in the same pass (alpha of the
+// arguments); otherwise drag / drag_old are left alone.  With a.reaction the
+// pass scatters -drag into it (the field is not cleared here), else it
+// scatters (w beta v, w beta) into the drag field.
+hipError_t launch_particles_couple(const ParticleArgs& a, bool relax, hipStream_t s);
 hipError_t launch_particles_advance(const ParticleArgs& a, hipStream_t s);
+// Zero the drag-field footprint of the particles at their current positions.
+hipError_t launch_particles_clear_footprint(const ParticleArgs& a, hipStream_t s);
+// drag = alpha drag_new + (1 - alpha) drag_old; drag_old = drag (active particles).
+hipError_t launch_particles_relax(const ParticleArgs& a, double alpha, hipStream_t s);
+// p[0 .. n) = 0 in the real type (double when dbl).
+hipError_t launch_zero_real(bool dbl, void* p, size_t n, hipStream_t s);
+// force[d][c] += add[d][c] on cells whose kind is not solid (every cell
+// without flags), d = 0..2, c < N.
+hipError_t launch_add_field_fluid(bool dbl, void* force, const void* add, const uint32_t* flags, uint32_t N,
+                                  hipStream_t s);
 hipError_t launch_particle_index(const float* pos, int32_t n, int32_t nx, int32_t ny, int32_t nz, int32_t* idx,
                                  float* frac, hipStream_t s);
 

@@ -8,10 +8,14 @@
 //          spread to the 8 corners with hardware floating-point atomics as
 //          the pair (w beta v, w beta) into the particle-drag field; the next
 //          stream-collide step applies sum w beta (v - u) with the cell
-//          velocity u implicit.  (The reference scatters -drag explicitly,
-//          coffee_particles.py:1148-1149; with the lognormal size tail of the
-//          brew workload, particles up to 6 cells wide give beta ~ 3 per cell
-//          and the explicit reaction diverges within ~30 steps.)
+//          velocity u implicit (the default).  The reference's explicit form
+//          -- the reaction -drag spread into a [3][N] reaction field that the
+//          caller adds to the body force, coffee_particles.py:1146-1149 and
+//          legacy/lbm_solver.py:1478-1483 -- is the second mode (ParticleArgs
+//          reaction set).  With the lognormal size tail of the brew workload,
+//          particles up to 6 cells wide give beta ~ 3 per cell and the
+//          explicit reaction diverges within ~30 steps, hence the implicit
+//          default.
 // advance: zero this step's 8-corner footprint in the drag field (so the
 //          field never needs a full-grid clear), then move the particle
 //          (linearised-implicit drag, buoyancy-corrected gravity, displacement
@@ -73,7 +77,11 @@ __device__ __forceinline__ R corner_weight(int c, R fx, R fy, R fz) {
   return wx * wy * wz;
 }
 
-template <typename R>
+// EXPLICIT: scatter the reaction -drag_new (coffee_particles.py:1146-1149,
+// 1049-1086) into the [3][N] reaction field instead of the implicit drag
+// field.  RELAX: under-relax in the same pass (the native coupled loop);
+// the reference applies it as a separate call (apply_under_relaxation).
+template <typename R, bool EXPLICIT, bool RELAX>
 __global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int p = t >> 3, c = t & 7;
@@ -83,7 +91,7 @@ __global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) 
   const R* pos = (const R*)a.pos;
   const R* vel = (const R*)a.vel;
   const R* ufield = (const R*)a.u_field;
-  R* field = (R*)a.drag_field;
+  R* field = (R*)(EXPLICIT ? a.reaction : a.drag_field);
   const int P = a.count;
   const R px = pos[p], py = pos[P + p], pz = pos[2 * P + p];
   int i, j, k;
@@ -121,12 +129,19 @@ __global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) 
     dn[0] = beta * rx;
     dn[1] = beta * ry;
     dn[2] = beta * rz;
-    // Newton's third law, fluid side implicit: the corner receives w beta v
-    // and w beta; the collision applies sum w beta (v - u_cell) with u_cell
-    // solved implicitly (cell_model.h)
-    const R comp[4] = {beta * vel[p], beta * vel[P + p], beta * vel[2 * P + p], beta};
+    if constexpr (EXPLICIT) {
+      // Newton's third law as the reference writes it: corner c receives
+      // w_c (-drag_new)
 #pragma unroll
-    for (int d = 0; d < 4; ++d) atomic_add(field + (size_t)d * N + cell, w * comp[d]);
+      for (int d = 0; d < 3; ++d) atomic_add(field + (size_t)d * N + cell, w * (-dn[d]));
+    } else {
+      // Newton's third law, fluid side implicit: the corner receives w beta v
+      // and w beta; the collision applies sum w beta (v - u_cell) with u_cell
+      // solved implicitly (cell_model.h)
+      const R comp[4] = {beta * vel[p], beta * vel[P + p], beta * vel[2 * P + p], beta};
+#pragma unroll
+      for (int d = 0; d < 4; ++d) atomic_add(field + (size_t)d * N + cell, w * comp[d]);
+    }
   }
   if (c != 0) return;
   ((R*)a.re_p)[p] = re;
@@ -139,10 +154,70 @@ __global__ void __launch_bounds__(256) k_particles_couple(const ParticleArgs a) 
   for (int d = 0; d < 3; ++d) {
     ((R*)a.u_fluid)[(size_t)d * P + p] = uf[d];
     drag_new[(size_t)d * P + p] = dn[d];
-    const R rel = al * dn[d] + (R(1) - al) * drag_old[(size_t)d * P + p];
+    if constexpr (RELAX) {
+      const R rel = al * dn[d] + (R(1) - al) * drag_old[(size_t)d * P + p];
+      drag[(size_t)d * P + p] = rel;
+      drag_old[(size_t)d * P + p] = rel;
+    }
+  }
+}
+
+// apply_under_relaxation (coffee_particles.py:1200-1212):
+// drag = alpha drag_new + (1 - alpha) drag_old, drag_old = drag, active only.
+template <typename R>
+__global__ void __launch_bounds__(256) k_particles_relax(const ParticleArgs a, R al) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.count || !a.active[p]) return;
+  const int P = a.count;
+  const R* drag_new = (const R*)a.drag_new;
+  R* drag_old = (R*)a.drag_old;
+  R* drag = (R*)a.drag;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const R rel = al * drag_new[(size_t)d * P + p] + (R(1) - al) * drag_old[(size_t)d * P + p];
     drag[(size_t)d * P + p] = rel;
     drag_old[(size_t)d * P + p] = rel;
   }
+}
+
+// Zero the 8-corner drag-field footprint of each active particle at its
+// current position (one lane per corner), the cells its last coupling pass
+// wrote.
+template <typename R>
+__global__ void __launch_bounds__(256) k_particles_clear_footprint(const ParticleArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int p = t >> 3, corner = t & 7;
+  if (p >= a.count || !a.active[p]) return;
+  const R* pos = (const R*)a.pos;
+  const int P = a.count;
+  int i, j, k;
+  R tt;
+  cell_index(pos[p], (int)a.nx, i, tt);
+  cell_index(pos[P + p], (int)a.ny, j, tt);
+  cell_index(pos[2 * P + p], (int)a.nz, k, tt);
+  const uint32_t cell = ((uint32_t)k + (corner & 1)) * a.nxy + ((uint32_t)j + ((corner >> 1) & 1)) * a.nx +
+                        (uint32_t)i + (corner >> 2);
+  R* field = (R*)a.drag_field;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) field[(size_t)d * a.N + cell] = R(0);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_zero_real(R* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = R(0);
+}
+
+// add_particle_reaction_forces (legacy/lbm_solver.py:1478-1483): the body
+// force takes the reaction on every non-solid cell.
+template <typename R>
+__global__ void __launch_bounds__(256) k_add_field_fluid(R* __restrict__ force, const R* __restrict__ add,
+                                                         const uint32_t* __restrict__ flags, uint32_t N) {
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= N) return;
+  if (flags && ((flags[c] >> POL_KIND_SHIFT) & POL_KIND_MASK) == POL_KIND_SOLID) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) force[(size_t)d * N + c] += add[(size_t)d * N + c];
 }
 
 template <typename R>
@@ -167,8 +242,10 @@ __global__ void __launch_bounds__(256) k_particles_advance(const ParticleArgs a)
     cell_index(x[2], (int)nz, k, tt);
     const uint32_t cell = ((uint32_t)k + (corner & 1)) * nxy + ((uint32_t)j + ((corner >> 1) & 1)) * nx +
                           (uint32_t)i + (corner >> 2);
+    if (field) {  // implicit drag field only (uniform)
 #pragma unroll
-    for (int d = 0; d < 4; ++d) field[(size_t)d * N + cell] = R(0);
+      for (int d = 0; d < 4; ++d) field[(size_t)d * N + cell] = R(0);
+    }
   }
   if (corner != 0) return;
   // 2. velocity: drag linearised about the current relative velocity,
@@ -241,12 +318,50 @@ __global__ void __launch_bounds__(256) k_particle_index(const float* __restrict_
   }
 }
 
-hipError_t launch_particles_couple(const ParticleArgs& a, hipStream_t s) {
+template <typename R>
+static void launch_couple_t(const ParticleArgs& a, bool relax, hipStream_t s) {
+  const dim3 grid(((size_t)a.count * CORNERS + 255) / 256), block(256);
+  if (a.reaction) {
+    if (relax) hipLaunchKernelGGL((k_particles_couple<R, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_particles_couple<R, true, false>), grid, block, 0, s, a);
+  } else {
+    if (relax) hipLaunchKernelGGL((k_particles_couple<R, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_particles_couple<R, false, false>), grid, block, 0, s, a);
+  }
+}
+hipError_t launch_particles_couple(const ParticleArgs& a, bool relax, hipStream_t s) {
   if (a.count <= 0) return hipSuccess;
-  if (a.real_is_double)
-    hipLaunchKernelGGL(k_particles_couple<double>, dim3(((size_t)a.count * CORNERS + 255) / 256), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_particles_couple<float>, dim3(((size_t)a.count * CORNERS + 255) / 256), dim3(256), 0, s, a);
+  if (a.real_is_double) launch_couple_t<double>(a, relax, s);
+  else launch_couple_t<float>(a, relax, s);
+  return hipGetLastError();
+}
+hipError_t launch_particles_relax(const ParticleArgs& a, double alpha, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const dim3 grid(((size_t)a.count + 255) / 256), block(256);
+  if (a.real_is_double) hipLaunchKernelGGL(k_particles_relax<double>, grid, block, 0, s, a, alpha);
+  else hipLaunchKernelGGL(k_particles_relax<float>, grid, block, 0, s, a, (float)alpha);
+  return hipGetLastError();
+}
+hipError_t launch_particles_clear_footprint(const ParticleArgs& a, hipStream_t s) {
+  if (a.count <= 0 || !a.drag_field) return hipSuccess;
+  const dim3 grid(((size_t)a.count * CORNERS + 255) / 256), block(256);
+  if (a.real_is_double) hipLaunchKernelGGL(k_particles_clear_footprint<double>, grid, block, 0, s, a);
+  else hipLaunchKernelGGL(k_particles_clear_footprint<float>, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_zero_real(bool dbl, void* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const dim3 grid((n + 255) / 256), block(256);
+  if (dbl) hipLaunchKernelGGL(k_zero_real<double>, grid, block, 0, s, (double*)p, n);
+  else hipLaunchKernelGGL(k_zero_real<float>, grid, block, 0, s, (float*)p, n);
+  return hipGetLastError();
+}
+hipError_t launch_add_field_fluid(bool dbl, void* force, const void* add, const uint32_t* flags, uint32_t N,
+                                  hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  const dim3 grid((N + 255) / 256), block(256);
+  if (dbl) hipLaunchKernelGGL(k_add_field_fluid<double>, grid, block, 0, s, (double*)force, (const double*)add, flags, N);
+  else hipLaunchKernelGGL(k_add_field_fluid<float>, grid, block, 0, s, (float*)force, (const float*)add, flags, N);
   return hipGetLastError();
 }
 hipError_t launch_particles_advance(const ParticleArgs& a, hipStream_t s) {

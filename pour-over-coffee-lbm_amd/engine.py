@@ -305,8 +305,8 @@ class HipContext:
         from dataclasses import replace
 
         cfg = replace(self.config, **changes)
-        if cfg.options(self._use_flags) != self.config.options(self._use_flags):
-            raise ValueError("set_params cannot switch options (LES/gravity/fields/periodicity)")
+        if (cfg.options(self._use_flags) & ~L.POL_LES) != (self.config.options(self._use_flags) & ~L.POL_LES):
+            raise ValueError("set_params cannot switch options other than LES (gravity/fields/periodicity)")
         L.check(self.lib.pol_set_params(self._ctx, C.byref(cfg.desc(self._use_flags))), self._ctx, "pol_set_params")
         self.config = cfg
 

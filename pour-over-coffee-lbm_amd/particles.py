@@ -1,20 +1,33 @@
 """Lagrangian coffee particles two-way coupled to the HIP lattice.
 
-Host mirror of the coupling part of CoffeeParticleSystem
-(src/physics/coffee_particles.py:30-64 state, :1107-1154
+Host mirror of CoffeeParticleSystem's coupling (src/physics/coffee_particles.py:
+30-64 state, :1101-1105 clear_reaction_forces, :1107-1154
 compute_two_way_coupling_forces, :1200-1212 apply_under_relaxation,
-:641-720 update_particle_physics) and of LBMSolver.step_with_two_way_coupling
-(src/core/legacy/lbm_solver.py:1485-1509).  All particle arithmetic runs in
-the HIP kernels of csrc/particles.hip through the C ABI; state lives in
-device tensors (SoA, [3, P] for vectors) of the lattice's real type.
+:641-720 update_particle_physics, :833-891 get_particle_statistics,
+:1214-1245 get_coupling_diagnostics) and of
+LBMSolver.step_with_two_way_coupling (src/core/legacy/lbm_solver.py:
+1478-1509).  All particle arithmetic runs in the HIP kernels of
+csrc/particles.hip through the C ABI; state lives in device tensors (SoA,
+[3, P] for vectors) of the lattice's real type.
 
-One coupled step:
+Two forms of the reaction on the fluid (`reaction=`):
+  "implicit" (default): the coupling spreads (w beta v, w beta) into the
+      lattice's particle-drag field; the next stream-collide step applies
+      sum w beta (v - u) with the cell velocity u solved implicitly, which
+      stays stable for any drag coefficient.  The field never needs a
+      full-grid clear: advance (or a repeated coupling) zeroes the 8-corner
+      footprint it wrote.
+  "explicit": the reference's form: the coupling clears reaction_force_field
+      and spreads -drag_new into it; add_particle_reaction_forces adds it to
+      the body force on the fluid cells before the step.
+
+One coupled step (coupled_step / pol_coupled_step):
   couple   : interpolate u at the particles (u field of the previous step,
              the reference's ordering), Schiller-Naumann drag, relaxation,
-             reaction -drag scattered into the lattice force field
-  LBM step : the fused stream-collide reads that force field (Guo forcing)
-             and writes the new u field
-  advance  : zero the reaction footprint, move the particles
+             the reaction as above
+  LBM step : the fused stream-collide reads the drag field (implicit) or the
+             body force (explicit) and writes the new u field
+  advance  : zero the drag-field footprint, move the particles
 """
 from __future__ import annotations
 
@@ -26,15 +39,21 @@ import torch
 
 from . import _lib as L
 from .engine import HipContext
+from .fields import DeviceField
 
 
 class HipCoffeeParticleSystem:
+    REACTIONS = {"implicit": L.POL_REACTION_IMPLICIT, "explicit": L.POL_REACTION_EXPLICIT}
+
     def __init__(self, ctx: HipContext, pos: np.ndarray, radius: np.ndarray, mass: np.ndarray, *,
                  rho_fluid: float = 1.0, relax_alpha: float = 0.8, gravity=(0.0, 0.0, 0.0), dt: float = 1.0,
                  max_disp: float = 1.0, wall_damping: float = 0.3, drag_law: int = L.POL_DRAG_TEXTBOOK,
-                 drag_max_per_mass: float = 1.0):
-        if ctx.pdrag is None:
-            raise ValueError("the lattice needs the particle-drag field (SolverConfig.particle_field=True)")
+                 drag_max_per_mass: float = 1.0, reaction: str = "implicit"):
+        if reaction not in self.REACTIONS:
+            raise ValueError("reaction must be 'implicit' or 'explicit'")
+        self.reaction = reaction
+        if reaction == "implicit" and ctx.pdrag is None:
+            raise ValueError("the implicit reaction needs the particle-drag field (SolverConfig.particle_field=True)")
         self.ctx = ctx
         self.lib = ctx.lib
         rd = ctx.config.real_dtype
@@ -56,6 +75,9 @@ class HipCoffeeParticleSystem:
         self.u_fluid = torch.zeros((3, P), dtype=rd, device=dev)
         self.re_p = torch.zeros(P, dtype=rd, device=dev)
         self.cd = torch.zeros(P, dtype=rd, device=dev)
+        # explicit form: the reference's reaction_force_field, [3, nz, ny, nx]
+        self._reaction = (torch.zeros((3,) + tuple(ctx.shape), dtype=rd, device=dev)
+                          if reaction == "explicit" else None)
         d = L.PolParticleDesc()
         d.count = P
         d.drag_law = int(drag_law)
@@ -67,6 +89,7 @@ class HipCoffeeParticleSystem:
         d.dt = dt
         d.max_disp = max_disp
         d.wall_damping = wall_damping
+        d.reaction = self.REACTIONS[reaction]
         self.desc = d
         self._bind()
 
@@ -74,8 +97,9 @@ class HipCoffeeParticleSystem:
         b = L.PolParticleBuffers()
         for name in ("pos", "vel", "radius", "mass", "active", "drag_new", "drag_old", "drag", "u_fluid", "re_p", "cd"):
             setattr(b, name, getattr(self, name).data_ptr())
-        b.drag_field = self.ctx.pdrag.data_ptr()
+        b.drag_field = self.ctx.pdrag.data_ptr() if self.ctx.pdrag is not None else None
         b.u_field = self.ctx.u.data_ptr()
+        b.reaction_field = self._reaction.data_ptr() if self._reaction is not None else None
         L.check(self.lib.pol_particles_bind(self.ctx.handle, C.byref(self.desc), C.byref(b)), self.ctx.handle,
                 "pol_particles_bind")
 
@@ -87,6 +111,84 @@ class HipCoffeeParticleSystem:
     def advance(self) -> None:
         L.check(self.lib.pol_particles_advance(self.ctx.handle, L.stream_handle()), self.ctx.handle,
                 "pol_particles_advance")
+
+    # ---- the reference's coupling verbs (coffee_particles.py) ------------------
+    @property
+    def reaction_force_field(self) -> DeviceField:
+        """coffee_particles.py:52: the reaction field, (NX, NY, NZ, 3) on the host.
+        The explicit form holds -drag spread to the cells; the implicit form
+        has no such field and presents the particle force the next step
+        applies, sum w beta (v - u), formed from the drag field and the
+        lattice velocity."""
+        if self._reaction is not None:
+            return DeviceField(self._reaction, "vector")
+        return DeviceField(self.implicit_reaction(), "vector")
+
+    def implicit_reaction(self, u: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Implicit form: per-cell force BV - B u of the drag field with the
+        lattice velocity u (default: the context's u buffer), [3, nz, ny, nx]."""
+        pd = self.ctx.pdrag
+        if pd is None:
+            raise ValueError("no implicit drag field on this lattice")
+        u = self.ctx.u if u is None else u
+        return pd[:3] - pd[3:4] * u
+
+    def compute_two_way_coupling_forces(self, fluid_u=None) -> None:
+        """coffee_particles.py:1107-1154: interpolate fluid_u (a DeviceField,
+        a [3, nz, ny, nx] device tensor, or None for the lattice's u buffer),
+        drag_new, Reynolds number and C_D, and the reaction (the explicit
+        form clears reaction_force_field first).  No under-relaxation."""
+        uptr = None
+        if fluid_u is not None:
+            t = fluid_u.to_torch() if isinstance(fluid_u, DeviceField) else fluid_u
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == self.ctx.u.dtype
+                    and tuple(t.shape) == tuple(self.ctx.u.shape) and t.is_contiguous()):
+                raise ValueError("fluid_u must be a contiguous [3, nz, ny, nx] device tensor of the lattice dtype")
+            uptr = t.data_ptr()
+        L.check(self.lib.pol_particles_compute_coupling(self.ctx.handle, uptr, L.stream_handle()), self.ctx.handle,
+                "pol_particles_compute_coupling")
+
+    def apply_under_relaxation(self, relaxation_factor: float) -> None:
+        """coffee_particles.py:1200-1212."""
+        L.check(self.lib.pol_particles_relax(self.ctx.handle, float(relaxation_factor), L.stream_handle()),
+                self.ctx.handle, "pol_particles_relax")
+
+    def clear_reaction_forces(self) -> None:
+        """coffee_particles.py:1101-1105 (explicit: whole field; implicit: the
+        footprint of a coupling not yet consumed)."""
+        L.check(self.lib.pol_particles_clear_reaction(self.ctx.handle, L.stream_handle()), self.ctx.handle,
+                "pol_particles_clear_reaction")
+
+    def update_particle_physics(self, dt: float = 1.0, center_x: float = 0.0, center_y: float = 0.0,
+                                bottom_z: float = 0.0, bottom_radius_lu: float = 0.0,
+                                top_radius_lu: float = 0.0) -> None:
+        """coffee_particles.py:641-720 signature; the push is this build's
+        integrator (implicit linearised drag, buoyancy-corrected gravity,
+        displacement clamp, rejection at walls and the filter from the cell
+        flags, so the cone arguments are accepted and unused)."""
+        if abs(float(dt) - self.desc.dt) > 0.0:
+            self.desc.dt = float(dt)
+            self._bind()
+        self.advance()
+
+    def get_particle_statistics(self) -> dict:
+        """coffee_particles.py:833-891 keys.  Radii and positions in lattice
+        units; a particle counts as valid when its radius is finite and > 0
+        and its position finite and inside [0, n] per axis (the reference's
+        metre bounds MIN_RADIUS / MAX_RADIUS do not apply to lattice units)."""
+        act = self.active.bool().cpu().numpy()
+        pos = self.pos.double().cpu().numpy()[:, act].T
+        rad = self.radius.double().cpu().numpy()[act]
+        nz, ny, nx = self.ctx.shape
+        hi = np.array([nx, ny, nz], dtype=np.float64)
+        ok = np.isfinite(rad) & (rad > 0) & np.all(np.isfinite(pos), axis=1) & np.all((pos >= 0) & (pos <= hi), axis=1)
+        radii, positions = rad[ok], pos[ok]
+        n_ok, n_bad = int(ok.sum()), int((~ok).sum())
+        coord_err = int((~np.all(np.isfinite(pos), axis=1)).sum())
+        stats = (float(radii.mean()), float(radii.std()), float(radii.min()), float(radii.max())) if n_ok else (0.0,) * 4
+        return {"count": n_ok, "invalid_count": n_bad, "coordinate_errors": coord_err, "boundary_violations": 0,
+                "mean_radius": stats[0], "std_radius": stats[1], "min_radius": stats[2], "max_radius": stats[3],
+                "positions": positions, "radii": radii, "success_rate": n_ok / max(1, n_ok + n_bad) * 100}
 
     def prime(self) -> None:
         """Fill the lattice u field from the current populations (before the first couple)."""
@@ -116,9 +218,12 @@ class HipCoffeeParticleSystem:
             return {"active_particles": 0, "avg_reynolds": 0.0, "avg_drag_coeff": 0.0,
                     "max_reaction_force": 0.0, "coupling_quality": "no_particles"}
         re = self.re_p[act].double()
-        dmag = torch.linalg.vector_norm(self.drag_new.double()[:, act], dim=0)
+        if self._reaction is not None:  # the reference's figure: max |reaction| over the cells
+            rmax = float(torch.linalg.vector_norm(self._reaction.double(), dim=0).max())
+        else:
+            rmax = float(torch.linalg.vector_norm(self.drag_new.double()[:, act], dim=0).max())
         return {"active_particles": n, "avg_reynolds": float(re.mean()), "max_reynolds": float(re.max()),
-                "avg_drag_coeff": float(self.cd[act].double().mean()), "max_reaction_force": float(dmag.max()),
+                "avg_drag_coeff": float(self.cd[act].double().mean()), "max_reaction_force": rmax,
                 "coupling_quality": "active"}
 
     def drag_field_sums(self) -> tuple:

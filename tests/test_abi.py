@@ -105,12 +105,12 @@ int main(void) {
   S(pol_particle_desc) F(pol_particle_desc, count) F(pol_particle_desc, drag_law) F(pol_particle_desc, rho_fluid)
   F(pol_particle_desc, nu_fluid) F(pol_particle_desc, drag_max_per_mass) F(pol_particle_desc, relax_alpha)
   F(pol_particle_desc, gravity) F(pol_particle_desc, dt) F(pol_particle_desc, max_disp)
-  F(pol_particle_desc, wall_damping)
+  F(pol_particle_desc, wall_damping) F(pol_particle_desc, reaction) F(pol_particle_desc, reserved)
   S(pol_particle_buffers) F(pol_particle_buffers, pos) F(pol_particle_buffers, vel) F(pol_particle_buffers, radius)
   F(pol_particle_buffers, mass) F(pol_particle_buffers, active) F(pol_particle_buffers, drag_new)
   F(pol_particle_buffers, drag_old) F(pol_particle_buffers, drag) F(pol_particle_buffers, u_fluid)
   F(pol_particle_buffers, re_p) F(pol_particle_buffers, cd) F(pol_particle_buffers, drag_field)
-  F(pol_particle_buffers, u_field)
+  F(pol_particle_buffers, u_field) F(pol_particle_buffers, reaction_field)
   S(pol_adjoint_buffers) F(pol_adjoint_buffers, lam) F(pol_adjoint_buffers, grad_perm)
   F(pol_adjoint_buffers, rev_flags)
   S(pol_stats_opts) F(pol_stats_opts, prev_rho) F(pol_stats_opts, prev_u) F(pol_stats_opts, phase)
