@@ -179,7 +179,8 @@ int pol_create(pol_ctx **out, const pol_desc *desc);
 /* Release the context and its scratch.  Replaces cleanup_backend
  * (compute_backends.py:443-452). */
 int pol_destroy(pol_ctx *ctx);
-/* Last error message of this context ("" if none). */
+/* Last error message of this context ("" if none); with ctx NULL, the message
+ * of the last failed context-free call (pol_reduce_noalloc) on this thread. */
 const char *pol_last_error(const pol_ctx *ctx);
 /* Attach caller-owned device buffers.  Replaces the memory-adapter attribute
  * contract f/f_new/rho/u/solid (src/core/backends/cpu_backend.py:207-212). */
@@ -223,6 +224,24 @@ int pol_macroscopic(pol_ctx *ctx, uint32_t what, double *red_out, void *stream);
 /* FP64 reductions of the current state; synchronous.  = pol_macroscopic
  * with POL_WANT_REDUCE. */
 int pol_reduce(pol_ctx *ctx, double *red_out, void *stream);
+
+/* ---- allocation-free reduction (no counterpart in the reference) ---------- */
+/* For callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc: the FP64
+ * reductions of pol_reduce (POL_RED_* slots) of a state described without a
+ * context.  desc and bufs as for pol_create / pol_bind; the state is
+ * bufs->f[0] (f[1] unused); red_out_dev: POL_NRED doubles of device memory,
+ * written asynchronously on `stream`.  Calls no hipMalloc / hipFree: the
+ * per-block partials live in a static device array of
+ * POL_NOALLOC_MAX_BLOCKS x POL_NRED doubles (6 MiB, which each GPU the
+ * library runs on holds until the process ends), so a state of more than
+ * POL_NOALLOC_MAX_BLOCKS x 256 work items (cells, or listed chunk lanes) is
+ * rejected with POL_ERR_INVALID.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (concurrent streams or threads) race and
+ * must be serialised by the caller.  On failure pol_last_error(NULL) gives
+ * the message. */
+#define POL_NOALLOC_MAX_BLOCKS 65536
+int pol_reduce_noalloc(const pol_desc *desc, const pol_buffers *bufs, double *red_out_dev, void *stream);
 
 /* ---- Lagrangian coffee particles (src/physics/coffee_particles.py) ------- */
 /* drag magnitude laws (C_D from the Schiller-Naumann piecewise form of

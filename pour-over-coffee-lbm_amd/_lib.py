@@ -19,6 +19,7 @@ LIB_PATH = PKG_DIR / "libpourover_hip.so"
 POL_ABI_VERSION = 2
 POL_Q = 19
 POL_NRED = 12
+POL_NOALLOC_MAX_BLOCKS = 65536
 POL_CHUNK = 32  # cells per launch chunk
 POL_RECORD_WORDS = 12  # uint32 per compact-layout chunk record
 POL_LAYOUT_NO_X_FACE = 1  # pol_buffers.layout_flags: no run reaches x = 0 or nx - 1
@@ -197,6 +198,7 @@ SIGNATURES = {
     "pol_step_from": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P]),
     "pol_macroscopic": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "pol_reduce": (C.c_int, [_P, _P, _P]),
+    "pol_reduce_noalloc": (C.c_int, [C.POINTER(PolDesc), C.POINTER(PolBuffers), _P, _P]),
     "pol_particles_bind": (C.c_int, [_P, C.POINTER(PolParticleDesc), C.POINTER(PolParticleBuffers)]),
     "pol_particles_couple": (C.c_int, [_P, _P]),
     "pol_particles_compute_coupling": (C.c_int, [_P, _P, _P]),

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -52,6 +53,8 @@ struct pol_ctx {
   std::string err;
 };
 
+static thread_local std::string g_free_err;  // message of the last failed context-free call
+
 static int fail(pol_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -99,16 +102,23 @@ int pol_device_count(int* count) {
   return n > 0 ? POL_OK : POL_ERR_NO_DEVICE;
 }
 
+// Grid, storage and physics checks of a descriptor (no HIP call).
+static bool desc_ok(const pol_desc& d) {
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return false;
+  const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
+  if (N >= (1ull << 31) || (uint64_t)d.nx * d.ny >= (1ull << 31)) return false;
+  if (storage_bytes(d.storage) == 0) return false;
+  if (!(d.tau > 0.5) || !std::isfinite(d.tau)) return false;
+  if ((d.options & POL_LES) && !(d.cs_smag >= 0.0)) return false;
+  return true;
+}
+
 int pol_create(pol_ctx** out, const pol_desc* desc) {
   if (!out || !desc) return POL_ERR_INVALID;
   *out = nullptr;
   const pol_desc& d = *desc;
-  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return POL_ERR_INVALID;
+  if (!desc_ok(d)) return POL_ERR_INVALID;
   const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
-  if (N >= (1ull << 31) || (uint64_t)d.nx * d.ny >= (1ull << 31)) return POL_ERR_INVALID;
-  if (storage_bytes(d.storage) == 0) return POL_ERR_INVALID;
-  if (!(d.tau > 0.5) || !std::isfinite(d.tau)) return POL_ERR_INVALID;
-  if ((d.options & POL_LES) && !(d.cs_smag >= 0.0)) return POL_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return POL_ERR_NO_DEVICE;
   if (d.device < 0 || d.device >= ndev) return POL_ERR_INVALID;
@@ -162,11 +172,17 @@ int pol_destroy(pol_ctx* c) {
   return POL_OK;
 }
 
-const char* pol_last_error(const pol_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* pol_last_error(const pol_ctx* c) {
+  if (c) return c->err.c_str();
+  return g_free_err.empty() ? "null context" : g_free_err.c_str();
+}
 
-int pol_bind(pol_ctx* c, const pol_buffers* b) {
-  if (!c || !b) return POL_ERR_INVALID;
-  if (!b->f[0] || !b->f[1] || b->f[0] == b->f[1]) return fail(c, POL_ERR_INVALID, "pol_bind: two distinct population buffers required");
+// Checks the buffers against the context's descriptor and records them
+// (no allocation, no HIP call).  pair: both population buffers required.
+static int bind_buffers(pol_ctx* c, const pol_buffers* b, bool pair) {
+  if (pair && (!b->f[0] || !b->f[1] || b->f[0] == b->f[1]))
+    return fail(c, POL_ERR_INVALID, "pol_bind: two distinct population buffers required");
+  if (!b->f[0]) return fail(c, POL_ERR_INVALID, "pol_bind: population buffer f[0] required");
   const uint32_t o = c->d.options;
   if ((o & POL_USE_FLAGS) && !b->cell_flags) return fail(c, POL_ERR_INVALID, "pol_bind: POL_USE_FLAGS without cell_flags");
   if ((o & POL_FORCE_FIELD) && !b->force) return fail(c, POL_ERR_INVALID, "pol_bind: POL_FORCE_FIELD without force buffer");
@@ -204,6 +220,11 @@ int pol_bind(pol_ctx* c, const pol_buffers* b) {
   c->cur = 0;
   c->err.clear();
   return POL_OK;
+}
+
+int pol_bind(pol_ctx* c, const pol_buffers* b) {
+  if (!c || !b) return POL_ERR_INVALID;
+  return bind_buffers(c, b, true);
 }
 
 int pol_set_params(pol_ctx* c, const pol_desc* d) {
@@ -368,6 +389,77 @@ int pol_macroscopic(pol_ctx* c, uint32_t what, double* red_out, void* stream) {
 
 int pol_reduce(pol_ctx* c, double* red_out, void* stream) {
   return pol_macroscopic(c, POL_WANT_REDUCE, red_out, stream);
+}
+
+// ---- allocation-free reduction --------------------------------------------
+// Per-block FP64 partials of pol_reduce_noalloc.  Each GPU the library runs on
+// holds its own copy of this array until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is a stand-in, not a
+// device address): it uses the per-device address hipGetSymbolAddress
+// returns, looked up once per device with that device current.
+static __device__ __align__(256) double scratch[(size_t)POL_NOALLOC_MAX_BLOCKS * POL_NRED];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+static int free_fail(int code, const std::string& msg) {
+  g_free_err = msg;
+  return code;
+}
+
+// The device address of `scratch` on the current device `dev`.
+static hipError_t scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+int pol_reduce_noalloc(const pol_desc* desc, const pol_buffers* bufs, double* red_out_dev, void* stream) {
+  if (!desc || !bufs || !red_out_dev) return free_fail(POL_ERR_INVALID, "pol_reduce_noalloc: null argument");
+  const pol_desc& d = *desc;
+  if (!desc_ok(d)) return free_fail(POL_ERR_INVALID, "pol_reduce_noalloc: invalid descriptor");
+  // the context on the stack holds descriptors only (its device pointers stay null)
+  pol_ctx c;
+  c.d = d;
+  const uint64_t N = (uint64_t)d.nx * d.ny * d.nz;
+  c.N = (uint32_t)N;
+  c.grid = make_grid(N);
+  c.wgrid = c.grid;
+  c.part_cap = POL_NOALLOC_MAX_BLOCKS;
+  const bool listed = bufs->active_chunks || bufs->chunk_records;
+  const uint32_t cc = bufs->chunk_cells ? bufs->chunk_cells : POL_CHUNK;
+  const uint64_t items = listed ? (uint64_t)bufs->n_chunks * cc : N;
+  if ((items + BLOCK - 1) / BLOCK > POL_NOALLOC_MAX_BLOCKS)
+    return free_fail(POL_ERR_INVALID, "pol_reduce_noalloc: " + std::to_string(items) +
+                                          " work items exceed the static scratch (at most " +
+                                          std::to_string((uint64_t)POL_NOALLOC_MAX_BLOCKS * BLOCK) + ")");
+  int st = bind_buffers(&c, bufs, false);
+  if (st) return free_fail(st, "pol_reduce_noalloc: " + c.err);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return free_fail(POL_ERR_NO_DEVICE, "pol_reduce_noalloc: no device");
+  if (d.device < 0 || d.device >= ndev || d.device >= kMaxDevices)
+    return free_fail(POL_ERR_INVALID, "pol_reduce_noalloc: device index out of range");
+  hipStream_t s = (hipStream_t)stream;
+  double* part = nullptr;
+  hipError_t e = hipSetDevice(d.device);
+  if (e == hipSuccess) e = scratch_address(d.device, &part);
+  if (e == hipSuccess) {
+    StepArgs a = make_args(&c, c.b.f[0], nullptr, POL_WANT_REDUCE);
+    a.partial = part;  // every block writes all of its POL_NRED partials
+    e = launch_stream_collide(d.storage, a, false, c.wgrid, s);
+  }
+  if (e == hipSuccess) e = launch_reduce_final(part, c.wgrid.nblk, red_out_dev, s);
+  if (e != hipSuccess)
+    return free_fail(POL_ERR_HIP, std::string("pol_reduce_noalloc: ") + hipGetErrorName(e) + " (" +
+                                      hipGetErrorString(e) + ")");
+  return POL_OK;
 }
 
 // ---- particles ----------------------------------------------------------

@@ -158,6 +158,11 @@ class HipContext:
 
     # -- lifecycle -----------------------------------------------------------
     def _bind(self) -> None:
+        L.check(self.lib.pol_bind(self._ctx, C.byref(self.buffers())), self._ctx, "pol_bind")
+
+    def buffers(self) -> L.PolBuffers:
+        """The pol_buffers of this context (f[0] / f[1] are the two ping-pong
+        buffers; the current state is f[self.current])."""
         b = L.PolBuffers()
         b.f[0] = self.f[0].data_ptr()
         b.f[1] = self.f[1].data_ptr()
@@ -177,7 +182,7 @@ class HipContext:
             b.n_chunks = self.chunks.numel()
             b.chunk_cells = self.chunk_cells
         b.particle_drag = self.pdrag.data_ptr() if self.pdrag is not None else None
-        L.check(self.lib.pol_bind(self._ctx, C.byref(b)), self._ctx, "pol_bind")
+        return b
 
     def _set_layout(self, flags: Optional[np.ndarray]) -> None:
         """Launch list and population layout for these flags: the compact

@@ -177,3 +177,27 @@ def test_missing_library_fails_loudly(tmp_path):
                        text=True, timeout=120)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "raised" in p.stdout
+
+
+def test_noalloc_reduction_rejects_before_any_device_call():
+    """pol_reduce_noalloc checks its arguments on the host: null arguments and a
+    state of more work items than the static scratch holds come back as
+    POL_ERR_INVALID with a message, before any HIP call (none is made here:
+    this runs without a device)."""
+    lib = L.load()
+    d = L.PolDesc()
+    d.nx = d.ny = d.nz = 512  # 134 217 728 cells: 524 288 blocks of 256
+    d.storage = L.POL_F32
+    d.tau = 0.6
+    b = L.PolBuffers()
+    b.f[0] = 0x1000  # never dereferenced: the size check comes first
+    out = C.c_void_p(0x2000)
+    assert lib.pol_reduce_noalloc(None, C.byref(b), out, None) == L.POL_ERR_INVALID
+    assert b"null argument" in lib.pol_last_error(None)
+    st = lib.pol_reduce_noalloc(C.byref(d), C.byref(b), out, None)
+    assert st == L.POL_ERR_INVALID
+    msg = lib.pol_last_error(None).decode()
+    assert "exceed the static scratch" in msg and str(L.POL_NOALLOC_MAX_BLOCKS * 256) in msg
+    d.tau = 0.4  # invalid physics
+    assert lib.pol_reduce_noalloc(C.byref(d), C.byref(b), out, None) == L.POL_ERR_INVALID
+    assert b"invalid descriptor" in lib.pol_last_error(None)
