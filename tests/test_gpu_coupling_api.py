@@ -125,7 +125,7 @@ def test_explicit_reaction_reference_sequence_vs_oracle(hip, oracle):
     np.testing.assert_allclose(tot, -dsum, rtol=1e-4, atol=1e-12)
     diag = ps.get_coupling_diagnostics()
     assert diag["active_particles"] == int(a.sum())
-    assert diag["max_reaction_force"] == pytest.approx(float(np.linalg.norm(react_g, axis=-1).max()), rel=1e-6)
+    assert diag["max_reaction_force"] == pytest.approx(float(np.linalg.norm(react_g, axis=0).max()), rel=1e-6)
     stats = ps.get_particle_statistics()
     assert stats["count"] == int(a.sum()) and stats["invalid_count"] == 0
     assert stats["positions"].shape == (int(a.sum()), 3)
