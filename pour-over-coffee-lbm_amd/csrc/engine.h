@@ -38,6 +38,7 @@ struct StepArgs {
   uint32_t N, nx, ny, nz, nxy;
   FastDiv dnx, dny;
   uint32_t per_xcd, nblk;
+  uint32_t order;          // block order of the stream-collide launches (ordered_block)
   uint32_t options;
   uint32_t what;
   double tau0, nu0, les_c, tau_max;

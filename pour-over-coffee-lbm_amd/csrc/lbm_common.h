@@ -129,9 +129,10 @@ __device__ __host__ inline uint32_t fdiv(uint32_t n, const FastDiv& f) {
 
 // ---------------------------------------------------------------------------
 // Launch geometry: 1-D grid of 256-thread blocks over the x-fastest cell
-// index.  Hardware deals workgroups round-robin to the 8 XCDs; the remap
-// gives each XCD one contiguous run of blocks so neighbouring x/y rows (and
-// the cache lines shared by misaligned +-x pulls) stay in one L2.
+// index (or the chunk list).  Hardware deals workgroups round-robin to the 8
+// XCDs.  logical_block gives each XCD one contiguous run of blocks, so
+// neighbouring rows stay in one L2 (the adjoint and diagnostics passes);
+// ordered_block selects the order per launch (stream-collide), see there.
 // ---------------------------------------------------------------------------
 constexpr int BLOCK = 256;
 constexpr int NXCD = 8;
@@ -151,6 +152,41 @@ inline Grid make_grid(uint64_t cells) {
 __device__ inline uint32_t logical_block(uint32_t per_xcd) {
   const uint32_t b = blockIdx.x;
   return (b % NXCD) * per_xcd + b / NXCD;
+}
+
+// Block order of a launch.  The stream-collide step reads and writes 38
+// population planes once each, so what counts is how many places of HBM the
+// chip touches at once:
+//   ORDER_LINEAR  block b works on logical block b: the whole chip sweeps one
+//                 front through every plane (dense 224^3 periodic step
+//                 0.306 ms, against 0.342-0.358 ms with XCD ranges, where 8
+//                 fronts run at once);
+//   k = 1..30     groups of 2^k consecutive logical blocks run on one XCD,
+//                 inside super-blocks of 8 * 2^k blocks (one front; each XCD
+//                 sees 2^k neighbouring blocks -- the compact layout's
+//                 chunks of a row and their shared lines -- in its own L2);
+//   ORDER_XCD_RANGES  logical_block above.
+// Measured on the 224^3 V60 step (compact layout, streaming stores), FP32 /
+// FP16 / FP64 ms: linear 0.1145 / 0.0623 / 0.2135, k = 2 0.1140 / 0.0609 /
+// 0.2141, XCD ranges 0.1185 / 0.0655 / 0.2322.  A step held in the
+// last-level cache (128^3) is faster with XCD ranges: neighbour rows then
+// hit the XCD's own L2.
+constexpr uint32_t ORDER_LINEAR = 0;
+constexpr uint32_t ORDER_XCD_RANGES = 31;
+__device__ inline uint32_t ordered_block(uint32_t order, uint32_t per_xcd) {
+  const uint32_t b = blockIdx.x;
+  if (order == ORDER_LINEAR) return b;
+  if (order == ORDER_XCD_RANGES) return (b % NXCD) * per_xcd + b / NXCD;
+  const uint32_t G = 1u << order, S = NXCD * G;
+  return (b / S) * S + (b % NXCD) * G + (b / NXCD) % G;
+}
+// Blocks to launch for an order: every logical block < g.nblk is covered;
+// the ones past it exit at once.
+inline uint32_t launch_blocks(uint32_t order, const Grid& g) {
+  if (order == ORDER_LINEAR) return g.nblk;
+  if (order == ORDER_XCD_RANGES) return g.launch;
+  const uint32_t S = (uint32_t)NXCD << order;
+  return (g.nblk + S - 1) / S * S;
 }
 
 // Plane-major buffer addressing.  Addr<true> keeps the element index

@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(BLOCK) k_init_equilibrium_sparse(const StepArg
                                                                    const typename Codec<S>::real* u) {
   using C = Codec<S>;
   using R = typename C::real;
-  const uint32_t lblk = logical_block(a.per_xcd);
+  const uint32_t lblk = ordered_block(a.order, a.per_xcd);
   if (lblk >= a.nblk) return;
   uint32_t cell, yz, x;
   int rb[9];
@@ -136,9 +136,9 @@ static hipError_t dispatch_sc(const StepArgs& a, bool store_f, const Grid& g, hi
   // 32-bit offsets when a population buffer is below 4 GiB; larger grids take
   // the general (FULL) kernel with 64-bit addressing
   if (fits_addr32(Q, a.P, sizeof(S)))
-    launch_sc<S, true>(a, store_f, flags, full, dim3(g.launch), s);
+    launch_sc<S, true>(a, store_f, flags, full, dim3(launch_blocks(a.order, g)), s);
   else
-    launch_sc<S, false>(a, store_f, flags, full, dim3(g.launch), s);
+    launch_sc<S, false>(a, store_f, flags, full, dim3(launch_blocks(a.order, g)), s);
   return hipGetLastError();
 }
 
@@ -162,7 +162,7 @@ template <typename S>
 static void launch_init(const StepArgs& a, const void* rho, const void* u, const Grid& g, hipStream_t s) {
   using R = typename Codec<S>::real;
   if (a.records) {
-    hipLaunchKernelGGL(k_init_equilibrium_sparse<S>, dim3(g.launch), dim3(BLOCK), 0, s, a, (const R*)rho,
+    hipLaunchKernelGGL(k_init_equilibrium_sparse<S>, dim3(launch_blocks(a.order, g)), dim3(BLOCK), 0, s, a, (const R*)rho,
                        (const R*)u);
   } else {
     hipLaunchKernelGGL(k_init_equilibrium<S>, dim3((a.N + 255) / 256), dim3(256), 0, s, (S*)a.fout, (const R*)rho,

@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   using R = typename C::real;
   using AD = Addr<A32>;
   using AI = typename AD::idx;
-  const uint32_t lblk = logical_block(a.per_xcd);
+  const uint32_t lblk = ordered_block(a.order, a.per_xcd);
   if (lblk >= a.nblk) return;  // whole padding block exits (uniform)
   uint32_t cell, yz, x;
   int rb[9];          // compact layout: slot bases of the nine neighbour rows
