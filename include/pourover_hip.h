@@ -65,6 +65,9 @@ enum { POL_F32 = 0, POL_F16 = 1, POL_BF16 = 2, POL_F64 = 3 };
 #define POL_PERM_FIELD 0x40u    /* per-cell permeability buffer (else darcy_K) */
 #define POL_GRAVITY 0x80u       /* constant body-force density gravity[3] */
 #define POL_PARTICLE_FIELD 0x100u /* implicit particle-drag field (pol_buffers.particle_drag) */
+#define POL_LES_FD 0x200u       /* the eddy viscosity of POL_LES (while it is on) from the reference's
+                                   finite differences (les_turbulence.py:318-380) instead of the
+                                   local-stress form, read from pol_buffers.nu_sgs (pol_les_viscosity) */
 
 /* cell flag word (uint32 per cell) */
 #define POL_LINK_MASK 0x7FFFFu  /* bit q: source x - e_q of link q is a wall -> halfway bounce-back */
@@ -154,6 +157,10 @@ typedef struct pol_buffers {
   const uint32_t *chunk_records;
   uint32_t stored_cells; /* plane stride of the compact layout */
   uint32_t layout_flags; /* POL_LAYOUT_* */
+  void *nu_sgs;          /* real [cells] eddy viscosity (POL_LES_FD): written by
+                            pol_les_viscosity, read by the steps */
+  const void *les_phase; /* real [cells] phase field of the |phi| < 0.9 interface
+                            suppression (POL_LES_FD), or NULL (phi = 1) */
 } pol_buffers;
 /* No run of the compact layout holds a cell at x = 0 or x = nx - 1: the
  * kernels then leave out the x-face wrap / clamp of the gather.  Set it only
@@ -224,6 +231,21 @@ int pol_macroscopic(pol_ctx *ctx, uint32_t what, double *red_out, void *stream);
 /* FP64 reductions of the current state; synchronous.  = pol_macroscopic
  * with POL_WANT_REDUCE. */
 int pol_reduce(pol_ctx *ctx, double *red_out, void *stream);
+
+/* ---- finite-difference eddy viscosity (src/physics/les_turbulence.py) ---- */
+/* nu_sgs = (C_s * 1)^2 |S|, |S| = sqrt(2 S:S), S from central differences of
+ * u_field (real [3][cells], NULL: the bound u buffer) over the interior
+ * cells, in float32 in the reference's operation order
+ * (_compute_sgs_from_vector, les_turbulence.py:318-380): zero where the cell
+ * flags switch LES off (POL_FLAG_NO_LES, the filter paper's mask), where
+ * |S| < 1e-3, where |phi| < 0.9 (les_phase), on the box faces, and clamped to
+ * nu_sgs_max (the reference's 0.1).  C_s is the descriptor's cs_smag.
+ * Needs POL_LES | POL_LES_FD.  With POL_LES_FD every pol_step_ex /
+ * pol_coupled_step step first forms nu_sgs from the bound u and then writes
+ * u (the velocity the legacy solver's next step reads,
+ * legacy/lbm_solver.py:856-857), and the collision uses
+ * tau = clamp(tau + 3 nu_sgs, 0.55, 1.90) (legacy/lbm_solver.py:565-572). */
+int pol_les_viscosity(pol_ctx *ctx, const void *u_field, void *stream);
 
 /* ---- allocation-free reduction (no counterpart in the reference) ---------- */
 /* For callers that cannot allocate device memory -- code under HIP graph
@@ -362,10 +384,20 @@ int pol_adjoint_bind(pol_ctx *ctx, const pol_adjoint_buffers *b);
  * dJ/dK accumulates in grad_perm, dJ/d(inlet_u) and J in the context (FP64).
  * Steps must be issued in reverse time order. */
 int pol_adjoint_step(pol_ctx *ctx, const void *f_state, int objective, void *stream);
+/* count reverse steps in one call: pol_adjoint_step with the forward states
+ * states + i * stride_bytes for i = count - 1 down to 0, the objective on for
+ * i >= objective_first. */
+int pol_adjoint_sweep(pol_ctx *ctx, const void *states, uint64_t stride_bytes, int count, int objective_first,
+                      void *stream);
+/* count - 1 fused steps in one call, each from states + (i - 1) * stride_bytes
+ * into states + i * stride_bytes (i = 1 .. count - 1): rebuilds a run of
+ * forward states from the first (the checkpointed reverse sweep; no
+ * counterpart in the reference). */
+int pol_step_chain(pol_ctx *ctx, void *states, uint64_t stride_bytes, int count, void *stream);
 /* Zero the adjoint state, grad_perm and the inlet-velocity gradient. */
 int pol_adjoint_reset(pol_ctx *ctx, void *stream);
-/* Copy the FP64 accumulators {dJ/du_in x, y, z, J} (4 doubles) to host;
- * synchronous. */
+/* Fold and copy the FP64 accumulators {dJ/du_in x, y, z, J} (4 doubles) to
+ * host; synchronous. */
 int pol_adjoint_collect(pol_ctx *ctx, double *out4, void *stream);
 
 /* ---- field diagnostics (src/core/numerical_stability.py,

@@ -76,7 +76,7 @@ typedef struct {
  * pd = particle-drag field values {sum w beta v (3), B = sum w beta} or NULL:
  * the particles exert BV - B u with u implicit. */
 static moments collide(real *g, unsigned kind, int porous, int les, real K, const real F0[3], const real *pd,
-                       const pol_desc *d) {
+                       const pol_desc *d, real nu_fd) {
   moments m;
   const real tau0 = (real)d->tau, nu0 = (real)((d->tau - 0.5) / 3.0);
   if (kind == POL_KIND_INLET) {
@@ -133,7 +133,13 @@ static moments collide(real *g, unsigned kind, int porous, int les, real K, cons
   real fe[19];
   for (int q = 0; q < 19; ++q) fe[q] = feq(q, rho, u[0], u[1], u[2]);
   real tau = tau0;
-  if (les && !porous) {
+  if (les && !porous && nu_fd >= 0) {
+    /* given eddy viscosity (the finite-difference form): tau_eff = tau + 3 nu,
+     * clamped to [0.55, 1.90] (legacy/lbm_solver.py:565-572) */
+    tau = tau0 + (real)3 * nu_fd;
+    if (tau < (real)0.55) tau = (real)0.55;
+    if (tau > (real)1.90) tau = (real)1.90;
+  } else if (les && !porous) {
     real P[3][3] = {{0}};
     for (int q = 0; q < 19; ++q) {
       int e[3] = {EX[q], EY[q], EZ[q]};
@@ -187,8 +193,19 @@ static void gather(const pol_desc *d, const uint32_t *flags, const real *fin, in
  * the collided state (solid cells: rho 1, u 0).  red_out (may be NULL)
  * receives the POL_NRED reductions (sums in FP64 in cell order).  pdrag
  * (may be NULL) is the [4][cells] particle-drag field. */
+int SFX(oracle_step_nu)(const pol_desc *d, const uint32_t *flags, const real *force, const real *perm,
+                        const real *pdrag, const real *nu, const real *fin, real *fout, real *rho_out, real *u_out,
+                        double *red_out);
 int SFX(oracle_step)(const pol_desc *d, const uint32_t *flags, const real *force, const real *perm, const real *pdrag,
                      const real *fin, real *fout, real *rho_out, real *u_out, double *red_out) {
+  return SFX(oracle_step_nu)(d, flags, force, perm, pdrag, NULL, fin, fout, rho_out, u_out, red_out);
+}
+
+/* One step with the eddy viscosity given per cell (nu, may be NULL: the
+ * local-stress form) when the descriptor has POL_LES and POL_LES_FD. */
+int SFX(oracle_step_nu)(const pol_desc *d, const uint32_t *flags, const real *force, const real *perm,
+                        const real *pdrag, const real *nu, const real *fin, real *fout, real *rho_out, real *u_out,
+                        double *red_out) {
   const size_t N = (size_t)d->nx * d->ny * d->nz;
   const int use_flags = (d->options & POL_USE_FLAGS) && flags;
   double red[POL_NRED];
@@ -229,7 +246,8 @@ int SFX(oracle_step)(const pol_desc *d, const uint32_t *flags, const real *force
         const int has_pd = (d->options & POL_PARTICLE_FIELD) && pdrag;
         if (has_pd)
           for (int i = 0; i < 4; ++i) pd[i] = pdrag[(size_t)i * N + cell];
-        moments m = collide(g, kind, porous, les, K, F0, has_pd ? pd : NULL, d);
+        const real nu_fd = (nu && (d->options & POL_LES_FD)) ? nu[cell] : (real)-1;
+        moments m = collide(g, kind, porous, les, K, F0, has_pd ? pd : NULL, d, nu_fd);
         if (fout)
           for (int q = 0; q < 19; ++q) fout[(size_t)q * N + cell] = g[q];
         if (rho_out) rho_out[cell] = m.rho;

@@ -21,6 +21,7 @@ LIB = HERE / "liboracle_lbm.so"
 POL_PERIODIC_X, POL_PERIODIC_Y, POL_PERIODIC_Z = 0x1, 0x2, 0x4
 POL_USE_FLAGS, POL_LES, POL_FORCE_FIELD, POL_PERM_FIELD, POL_GRAVITY = 0x8, 0x10, 0x20, 0x40, 0x80
 POL_PARTICLE_FIELD = 0x100
+POL_LES_FD = 0x200
 NRED = 12
 
 
@@ -104,9 +105,11 @@ def _sfx(prec):
     return "_f64" if prec == "f64" else "_f32"
 
 
-def step(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, want_macro=False, prec="f64", pdrag=None):
+def step(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, want_macro=False, prec="f64", pdrag=None,
+         nu=None):
     """One oracle step.  Returns (f_new, rho, u, red) (rho/u None unless want_macro).
-    pdrag: [4, nz, ny, nx] particle-drag field (used when desc has POL_PARTICLE_FIELD)."""
+    pdrag: [4, nz, ny, nx] particle-drag field (used when desc has POL_PARTICLE_FIELD).
+    nu: [nz, ny, nx] given eddy viscosity (used when desc has POL_LES and POL_LES_FD)."""
     dt = _dtype(prec)
     f = np.ascontiguousarray(f, dtype=dt)
     fout = np.zeros_like(f)
@@ -118,9 +121,10 @@ def step(desc: Desc, f: np.ndarray, flags=None, force=None, perm=None, want_macr
     fo = np.ascontiguousarray(force, dtype=dt) if force is not None else None
     pm = np.ascontiguousarray(perm, dtype=dt) if perm is not None else None
     pd = np.ascontiguousarray(pdrag, dtype=dt) if pdrag is not None else None
-    fn = getattr(lib(), "oracle_step" + _sfx(prec))
+    nv = np.ascontiguousarray(nu, dtype=dt) if nu is not None else None
+    fn = getattr(lib(), "oracle_step_nu" + _sfx(prec))
     fn.restype = C.c_int
-    rc = fn(C.byref(desc), _p(fl), _p(fo), _p(pm), _p(pd), _p(f), _p(fout), _p(rho), _p(u), _p(red))
+    rc = fn(C.byref(desc), _p(fl), _p(fo), _p(pm), _p(pd), _p(nv), _p(f), _p(fout), _p(rho), _p(u), _p(red))
     if rc != 0:
         raise RuntimeError("oracle_step failed")
     return fout, rho, u, red

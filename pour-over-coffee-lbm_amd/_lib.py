@@ -34,6 +34,7 @@ POL_FORCE_FIELD = 0x20
 POL_PERM_FIELD = 0x40
 POL_GRAVITY = 0x80
 POL_PARTICLE_FIELD = 0x100
+POL_LES_FD = 0x200
 POL_LINK_MASK = 0x7FFFF
 POL_KIND_SHIFT = 24
 POL_KIND_MASK = 0x7
@@ -122,6 +123,7 @@ class PolBuffers(C.Structure):
         ("active_chunks", C.c_void_p), ("n_chunks", C.c_uint32), ("chunk_cells", C.c_uint32),
         ("particle_drag", C.c_void_p),
         ("chunk_records", C.c_void_p), ("stored_cells", C.c_uint32), ("layout_flags", C.c_uint32),
+        ("nu_sgs", C.c_void_p), ("les_phase", C.c_void_p),
     ]
 
 
@@ -199,6 +201,7 @@ SIGNATURES = {
     "pol_macroscopic": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "pol_reduce": (C.c_int, [_P, _P, _P]),
     "pol_reduce_noalloc": (C.c_int, [C.POINTER(PolDesc), C.POINTER(PolBuffers), _P, _P]),
+    "pol_les_viscosity": (C.c_int, [_P, _P, _P]),
     "pol_particles_bind": (C.c_int, [_P, C.POINTER(PolParticleDesc), C.POINTER(PolParticleBuffers)]),
     "pol_particles_couple": (C.c_int, [_P, _P]),
     "pol_particles_compute_coupling": (C.c_int, [_P, _P, _P]),
@@ -211,6 +214,8 @@ SIGNATURES = {
     "pol_adjoint_bind": (C.c_int, [_P, C.POINTER(PolAdjointBuffers)]),
     "pol_adjoint_step": (C.c_int, [_P, _P, C.c_int, _P]),
     "pol_adjoint_reset": (C.c_int, [_P, _P]),
+    "pol_adjoint_sweep": (C.c_int, [_P, _P, C.c_uint64, C.c_int, C.c_int, _P]),
+    "pol_step_chain": (C.c_int, [_P, _P, C.c_uint64, C.c_int, _P]),
     "pol_adjoint_collect": (C.c_int, [_P, _P, _P]),
     "pol_field_stats": (C.c_int, [_P, C.POINTER(PolStatsOpts), _P, _P, _P]),
     "pol_export_fields": (C.c_int, [_P, _P, _P, _P]),

@@ -110,6 +110,7 @@ static bool desc_ok(const pol_desc& d) {
   if (storage_bytes(d.storage) == 0) return false;
   if (!(d.tau > 0.5) || !std::isfinite(d.tau)) return false;
   if ((d.options & POL_LES) && !(d.cs_smag >= 0.0)) return false;
+  if ((d.options & POL_LES_FD) && !(d.nu_sgs_max >= 0.0)) return false;
   return true;
 }
 
@@ -189,6 +190,7 @@ static int bind_buffers(pol_ctx* c, const pol_buffers* b, bool pair) {
   if ((o & POL_PERM_FIELD) && !b->perm) return fail(c, POL_ERR_INVALID, "pol_bind: POL_PERM_FIELD without perm buffer");
   if ((o & POL_PARTICLE_FIELD) && !b->particle_drag)
     return fail(c, POL_ERR_INVALID, "pol_bind: POL_PARTICLE_FIELD without particle_drag buffer");
+  if ((o & POL_LES_FD) && !b->nu_sgs) return fail(c, POL_ERR_INVALID, "pol_bind: POL_LES_FD without nu_sgs buffer");
   if ((o & POL_PERM_FIELD) && !(o & POL_USE_FLAGS)) return fail(c, POL_ERR_INVALID, "pol_bind: permeability needs cell flags");
   const uint32_t cc = b->chunk_cells ? b->chunk_cells : POL_CHUNK;
   if (cc != 8 && cc != 16 && cc != 32 && cc != 64)
@@ -274,6 +276,7 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.force = c->b.force;
   a.perm = c->b.perm;
   a.pdrag = c->b.particle_drag;
+  a.nu_sgs = c->b.nu_sgs;
   a.rho = c->b.rho;
   a.u = c->b.u;
   a.partial = c->partial;
@@ -343,16 +346,55 @@ static int finish_reduce(pol_ctx* c, double* red_out, hipStream_t s, const char*
   return e == hipSuccess ? POL_OK : hip_fail(c, e, fn);
 }
 
+// The reference's finite-difference eddy viscosity from the velocity u
+// (real [3][cells]) into the bound nu_sgs buffer.
+static hipError_t les_fd_pass(const pol_ctx* c, const void* u, hipStream_t s) {
+  const pol_desc& d = c->d;
+  LesFdArgs a{};
+  a.u = u;
+  a.flags = (d.options & POL_USE_FLAGS) ? c->b.cell_flags : nullptr;
+  a.phase = c->b.les_phase;
+  a.nu = c->b.nu_sgs;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.nz = d.nz;
+  a.nxy = (uint32_t)d.nx * d.ny;
+  a.N = c->N;
+  a.dnx = make_fastdiv(d.nx);
+  a.dny = make_fastdiv(d.ny);
+  a.cs2 = (float)(d.cs_smag * d.cs_smag);  // (C_s * filter width 1)^2
+  a.nu_max = (float)d.nu_sgs_max;
+  return launch_les_fd(d.storage == POL_F64, a, s);
+}
+
+static bool les_fd(const pol_ctx* c) { return (c->d.options & (POL_LES | POL_LES_FD)) == (POL_LES | POL_LES_FD); }
+
+int pol_les_viscosity(pol_ctx* c, const void* u_field, void* stream) {
+  if (!c) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_les_viscosity: buffers not bound");
+  if (!les_fd(c)) return fail(c, POL_ERR_STATE, "pol_les_viscosity: needs POL_LES | POL_LES_FD");
+  const void* u = u_field ? u_field : c->b.u;
+  if (!u) return fail(c, POL_ERR_INVALID, "pol_les_viscosity: no velocity field");
+  hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = les_fd_pass(c, u, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_les_viscosity");
+}
+
 int pol_step_ex(pol_ctx* c, int nsteps, uint32_t what, double* red_out, void* stream) {
   if (!c || nsteps < 0) return POL_ERR_INVALID;
   if (!c->bound) return fail(c, POL_ERR_STATE, "pol_step: buffers not bound");
-  int st = check_what(c, what, red_out, "pol_step");
+  const bool fd = les_fd(c);
+  int st = check_what(c, what | (fd ? POL_WANT_MACRO : 0u), red_out, "pol_step");
   if (st) return st;
   if (nsteps == 0) return POL_OK;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(c->d.device);
   for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
-    const uint32_t w = (i == nsteps - 1) ? what : 0u;
+    // finite-difference LES: nu_sgs from the u the previous step wrote, and
+    // this step writes u for the next (legacy/lbm_solver.py:856-861)
+    if (fd) e = les_fd_pass(c, c->b.u, s);
+    if (e != hipSuccess) break;
+    const uint32_t w = ((i == nsteps - 1) ? what : 0u) | (fd ? POL_WANT_MACRO : 0u);
     StepArgs a = make_args(c, c->b.f[c->cur], c->b.f[1 - c->cur], w);
     e = launch_stream_collide(c->d.storage, a, true, c->wgrid, s);
     if (e == hipSuccess) c->cur = 1 - c->cur;
@@ -367,6 +409,7 @@ int pol_step(pol_ctx* c, int nsteps, void* stream) { return pol_step_ex(c, nstep
 int pol_step_from(pol_ctx* c, const void* fin, void* fout, uint32_t what, double* red_out, void* stream) {
   if (!c || !fin || !fout || fin == fout) return POL_ERR_INVALID;
   if (!c->bound) return fail(c, POL_ERR_STATE, "pol_step_from: buffers not bound");
+  if (les_fd(c)) return fail(c, POL_ERR_UNSUPPORTED, "pol_step_from: not with the finite-difference LES");
   int st = check_what(c, what, red_out, "pol_step_from");
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
@@ -635,8 +678,10 @@ int pol_coupled_step(pol_ctx* c, int nsteps, uint32_t what, double* red_out, voi
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(c->d.device);
   const bool dbl = c->d.storage == POL_F64;
+  const bool fd = les_fd(c);
   for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
     e = couple_pass(c, c->pa, true, s);
+    if (e == hipSuccess && fd) e = les_fd_pass(c, c->b.u, s);
     if (e == hipSuccess && expl) {
       // legacy/lbm_solver.py:1488,1499: clear_body_force, then add the reaction
       e = launch_zero_real(dbl, c->b.force, 3 * (size_t)c->N, s);
@@ -671,6 +716,8 @@ int pol_adjoint_bind(pol_ctx* c, const pol_adjoint_buffers* b) {
     return fail(c, POL_ERR_UNSUPPORTED, "pol_adjoint_bind: adjoint supports F32 and F64 storage");
   if (c->d.options & POL_PARTICLE_FIELD)
     return fail(c, POL_ERR_UNSUPPORTED, "pol_adjoint_bind: the adjoint does not cover the particle-drag field");
+  if (c->d.options & POL_LES_FD)
+    return fail(c, POL_ERR_UNSUPPORTED, "pol_adjoint_bind: the adjoint covers the local-stress LES only");
   if (!(c->d.options & POL_USE_FLAGS) || !c->bound)
     return fail(c, POL_ERR_STATE, "pol_adjoint_bind: bind forward buffers with cell flags first");
   if (!b->lam[0] || !b->lam[1] || b->lam[0] == b->lam[1] || !b->rev_flags)
@@ -690,15 +737,14 @@ int pol_adjoint_reset(pol_ctx* c, void* stream) {
   if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[0], 0, (size_t)POL_Q * c->P * rb, s);
   if (e == hipSuccess) e = hipMemsetAsync(c->ab.lam[1], 0, (size_t)POL_Q * c->P * rb, s);
   if (e == hipSuccess && c->ab.grad_perm) e = hipMemsetAsync(c->ab.grad_perm, 0, (size_t)c->N * sizeof(double), s);
+  if (e == hipSuccess) e = hipMemsetAsync(c->adj_partial, 0, (size_t)c->wgrid.nblk * 4 * sizeof(double), s);
   if (e == hipSuccess) e = hipMemsetAsync(c->adj_acc, 0, 4 * sizeof(double), s);
   c->acur = 0;
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_adjoint_reset");
 }
 
-int pol_adjoint_step(pol_ctx* c, const void* f_state, int objective, void* stream) {
-  if (!c || !f_state) return POL_ERR_INVALID;
-  if (!c->adjoint) return fail(c, POL_ERR_STATE, "pol_adjoint_step: adjoint not bound");
-  hipStream_t s = (hipStream_t)stream;
+// One reverse step on stream s (the device is current).
+static hipError_t adjoint_launch(pol_ctx* c, const void* f_state, int objective, hipStream_t s) {
   AdjointArgs A{};
   A.fw = make_args(c, f_state, nullptr, 0u);
   A.lam_in = c->ab.lam[c->acur];
@@ -706,12 +752,44 @@ int pol_adjoint_step(pol_ctx* c, const void* f_state, int objective, void* strea
   A.grad_perm = (c->d.options & POL_PERM_FIELD) ? c->ab.grad_perm : nullptr;
   A.partial = c->adj_partial;
   A.objective = objective;
+  const hipError_t e = launch_adjoint(c->d.storage, A, c->ab.rev_flags, c->wgrid, s);
+  if (e == hipSuccess) c->acur = 1 - c->acur;
+  return e;
+}
+
+int pol_adjoint_step(pol_ctx* c, const void* f_state, int objective, void* stream) {
+  if (!c || !f_state) return POL_ERR_INVALID;
+  if (!c->adjoint) return fail(c, POL_ERR_STATE, "pol_adjoint_step: adjoint not bound");
   hipError_t e = hipSetDevice(c->d.device);
-  if (e == hipSuccess) e = launch_adjoint(c->d.storage, A, c->ab.rev_flags, c->wgrid, s);
-  if (e == hipSuccess) e = launch_adjoint_final(c->adj_partial, c->wgrid.nblk, c->adj_acc, s);
-  if (e != hipSuccess) return hip_fail(c, e, "pol_adjoint_step");
-  c->acur = 1 - c->acur;
-  return POL_OK;
+  if (e == hipSuccess) e = adjoint_launch(c, f_state, objective, (hipStream_t)stream);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_adjoint_step");
+}
+
+int pol_adjoint_sweep(pol_ctx* c, const void* states, uint64_t stride_bytes, int count, int objective_first,
+                      void* stream) {
+  if (!c || !states || count < 0) return POL_ERR_INVALID;
+  if (!c->adjoint) return fail(c, POL_ERR_STATE, "pol_adjoint_sweep: adjoint not bound");
+  if (count > 1 && stride_bytes == 0) return fail(c, POL_ERR_INVALID, "pol_adjoint_sweep: zero state stride");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  for (int i = count - 1; i >= 0 && e == hipSuccess; --i)
+    e = adjoint_launch(c, (const char*)states + (uint64_t)i * stride_bytes, i >= objective_first ? 1 : 0, s);
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_adjoint_sweep");
+}
+
+int pol_step_chain(pol_ctx* c, void* states, uint64_t stride_bytes, int count, void* stream) {
+  if (!c || !states || count < 0) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_step_chain: buffers not bound");
+  if (les_fd(c)) return fail(c, POL_ERR_UNSUPPORTED, "pol_step_chain: not with the finite-difference LES");
+  if (count > 1 && stride_bytes == 0) return fail(c, POL_ERR_INVALID, "pol_step_chain: zero state stride");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  for (int i = 1; i < count && e == hipSuccess; ++i) {
+    const StepArgs a = make_args(c, (const char*)states + (uint64_t)(i - 1) * stride_bytes,
+                                 (char*)states + (uint64_t)i * stride_bytes, 0u);
+    e = launch_stream_collide(c->d.storage, a, true, c->wgrid, s);
+  }
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_step_chain");
 }
 
 int pol_adjoint_collect(pol_ctx* c, double* out4, void* stream) {
@@ -719,6 +797,7 @@ int pol_adjoint_collect(pol_ctx* c, double* out4, void* stream) {
   if (!c->adjoint) return fail(c, POL_ERR_STATE, "pol_adjoint_collect: adjoint not bound");
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(c->d.device);
+  if (e == hipSuccess) e = launch_adjoint_final(c->adj_partial, c->wgrid.nblk, c->adj_acc, s);
   if (e == hipSuccess) e = hipMemcpyAsync(out4, c->adj_acc, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_adjoint_collect");

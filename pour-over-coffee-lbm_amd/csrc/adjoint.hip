@@ -42,24 +42,43 @@ struct LamMoments {
     oz = xz * x + yz * y + zz * z;
   }
 };
+// Pairwise over opposite directions (w_a = w_b, e_b = -e_a): the even
+// moments take w (lam_a + lam_b), the odd ones w (lam_a - lam_b).
 template <typename R>
 __device__ inline LamMoments<R> lam_moments(const R (&lam)[Q]) {
   LamMoments<R> M{};
-  static_for<0, Q>([&](auto qc) {
-    constexpr int p = decltype(qc)::value;
-    const R lw = lam[p] * W<R>(p);
-    M.m0 += lw;
-    eacc<EX(p)>(M.m1x, lw);
-    eacc<EY(p)>(M.m1y, lw);
-    eacc<EZ(p)>(M.m1z, lw);
-    eacc<EX(p) * EX(p)>(M.xx, lw);
-    eacc<EY(p) * EY(p)>(M.yy, lw);
-    eacc<EZ(p) * EZ(p)>(M.zz, lw);
-    eacc<EX(p) * EY(p)>(M.xy, lw);
-    eacc<EX(p) * EZ(p)>(M.xz, lw);
-    eacc<EY(p) * EZ(p)>(M.yz, lw);
+  M.m0 = lam[0] * W<R>(0);
+  static_for<0, 9>([&](auto pc) {
+    constexpr int p = PAIR_A(decltype(pc)::value);
+    constexpr int o = OPP(p);
+    const R ws = (lam[p] + lam[o]) * W<R>(p);
+    const R wd = (lam[p] - lam[o]) * W<R>(p);
+    M.m0 += ws;
+    eacc<EX(p)>(M.m1x, wd);
+    eacc<EY(p)>(M.m1y, wd);
+    eacc<EZ(p)>(M.m1z, wd);
+    eacc<EX(p) * EX(p)>(M.xx, ws);
+    eacc<EY(p) * EY(p)>(M.yy, ws);
+    eacc<EZ(p) * EZ(p)>(M.zz, ws);
+    eacc<EX(p) * EY(p)>(M.xy, ws);
+    eacc<EX(p) * EZ(p)>(M.xz, ws);
+    eacc<EY(p) * EZ(p)>(M.yz, ws);
   });
   return M;
+}
+
+// FP32: the hardware reciprocal and square root (1 ulp): the adjoint is
+// linearised about the recomputed state, where a last-bit difference is
+// immaterial; FP64 keeps the correctly rounded operations.
+template <typename R>
+__device__ __forceinline__ R arcp(R x) {
+  if constexpr (std::is_same_v<R, float>) return __builtin_amdgcn_rcpf(x);
+  else return R(1) / x;
+}
+template <typename R>
+__device__ __forceinline__ R asqrt(R x) {
+  if constexpr (std::is_same_v<R, float>) return __builtin_amdgcn_sqrtf(x);
+  else return sqrt(x);
 }
 // sum_p lam_p d feq_p(rho, u)/du = rho (3 M1 + 9 M2 u - 3 u M0)
 template <typename R>
@@ -70,6 +89,18 @@ __device__ inline void eq_grad_u(const LamMoments<R>& M, R rho, R ux, R uy, R uz
   ay = rho * (R(3) * M.m1y + R(9) * ty - R(3) * uy * M.m0);
   az = rho * (R(3) * M.m1z + R(9) * tz - R(3) * uz * M.m0);
 }
+
+// The bounce bits of a flag word moved to the opposite directions: bit p of
+// the result is bit opp(p) of fl (bits 1..18; the pairs (1,2) (3,4) (5,6)
+// (8,9) (12,13) (16,17) are one bit apart, (7,10) (11,14) (15,18) three).
+__device__ __forceinline__ uint32_t opposite_links(uint32_t fl) {
+  constexpr uint32_t LO1 = (1u << 1) | (1u << 3) | (1u << 5) | (1u << 8) | (1u << 12) | (1u << 16);
+  constexpr uint32_t LO3 = (1u << 7) | (1u << 11) | (1u << 15);
+  return ((fl & LO1) << 1) | ((fl >> 1) & LO1) | ((fl & LO3) << 3) | ((fl >> 3) & LO3);
+}
+static_assert(OPP(1) == 2 && OPP(3) == 4 && OPP(5) == 6 && OPP(8) == 9 && OPP(12) == 13 && OPP(16) == 17 &&
+                  OPP(7) == 10 && OPP(11) == 14 && OPP(15) == 18,
+              "opposite_links assumes the config/core.py direction order");
 
 struct AdjGeom {
   uint32_t x, y, z;
@@ -129,12 +160,13 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
     const R* __restrict__ mu_in = (const R*)A.lam_in;
     // ---- 1. lambda_p = dJ/df^{n+1}_p(x), reverse gather of mu^{n+1} ----
     if constexpr (SPARSE) {
-      // one load per direction from the selected reader (compact_gather.h)
+      // one load per direction from the selected reader (compact_gather.h);
+      // bit p of `readers`: f_p(x) has a reader (normal link or x's own bounce
+      // link opp p), else lambda_p = 0
+      const uint32_t readers = rf | (kind != POL_KIND_INLET ? opposite_links(fl) : 0u);
       compact_readers<R, A32, XF>(mu_in, P, rb, x, a.nx, px, own, rf, [&](auto pc, R v) {
         constexpr int p = decltype(pc)::value;
-        bool bb = false;
-        if constexpr (p != 0) bb = kind != POL_KIND_INLET && (fl & (1u << OPP(p)));
-        lam[p] = ((rf & (1u << p)) || bb) ? v : R(0);
+        lam[p] = (readers & (1u << p)) ? v : R(0);
       });
     } else static_for<0, Q>([&](auto qc) {
       constexpr int p = decltype(qc)::value;
@@ -233,28 +265,24 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
     } else {
       // reduce the gathered deviations to what the adjoint needs: moments,
       // the second moment (LES stress) and sum_q lam_q D_q
-      R s36 = 0, mx = 0, my = 0, mz = 0, lamD = 0;
+      // pairwise: even moments from d_a + d_b, odd ones from d_a - d_b
+      R s36 = dv[0], mx = 0, my = 0, mz = 0, lamD = lam[0] * dv[0];
       R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const R d = dv[q];
-        s36 += d;
-        lamD += lam[q] * d;
-        if constexpr (EX(q) > 0) mx += d;
-        if constexpr (EX(q) < 0) mx -= d;
-        if constexpr (EY(q) > 0) my += d;
-        if constexpr (EY(q) < 0) my -= d;
-        if constexpr (EZ(q) > 0) mz += d;
-        if constexpr (EZ(q) < 0) mz -= d;
-        if constexpr (EX(q) != 0) pxx += d;
-        if constexpr (EY(q) != 0) pyy += d;
-        if constexpr (EZ(q) != 0) pzz += d;
-        if constexpr (EX(q) * EY(q) > 0) pxy += d;
-        if constexpr (EX(q) * EY(q) < 0) pxy -= d;
-        if constexpr (EX(q) * EZ(q) > 0) pxz += d;
-        if constexpr (EX(q) * EZ(q) < 0) pxz -= d;
-        if constexpr (EY(q) * EZ(q) > 0) pyz += d;
-        if constexpr (EY(q) * EZ(q) < 0) pyz -= d;
+      static_for<0, 9>([&](auto pc) {
+        constexpr int q = PAIR_A(decltype(pc)::value);
+        constexpr int o = OPP(q);
+        const R ds = dv[q] + dv[o], dd = dv[q] - dv[o];
+        s36 += ds;
+        lamD += lam[q] * dv[q] + lam[o] * dv[o];
+        eacc<EX(q)>(mx, dd);
+        eacc<EY(q)>(my, dd);
+        eacc<EZ(q)>(mz, dd);
+        eacc<EX(q) * EX(q)>(pxx, ds);
+        eacc<EY(q) * EY(q)>(pyy, ds);
+        eacc<EZ(q) * EZ(q)>(pzz, ds);
+        eacc<EX(q) * EY(q)>(pxy, ds);
+        eacc<EX(q) * EZ(q)>(pxz, ds);
+        eacc<EY(q) * EZ(q)>(pyz, ds);
       });
       // the linearisation point of the adjoint: a rounded 1/36 is immaterial
       const R i36 = R(1.0 / 36.0);
@@ -262,7 +290,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
       mx *= i36;
       my *= i36;
       mz *= i36;
-      const R inv_rho = R(1) / rho;
+      const R inv_rho = arcp(rho);
       R F0x = G0, F0y = G1, F0z = G2;
       if (a.options & POL_FORCE_FIELD) {
         const R* Ff = (const R*)a.force;
@@ -293,19 +321,20 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         R ux = vx, uy = vy, uz = vz, Fx = F0x, Fy = F0y, Fz = F0z;
         R dk = 0, fk = 0, umag = 0, sK = 0, c0 = 0, c1 = 0, vmag = 0, rr = 0, D = 0;
         if (porous) {
-          sK = sqrt(K);
-          c0 = R(1) + nu0 / (R(2) * K);
-          c1 = (R)a.CF / (R(2) * sK);
-          vmag = sqrt(vx * vx + vy * vy + vz * vz);
-          rr = sqrt(c0 * c0 + R(4) * c1 * vmag);
+          sK = asqrt(K);
+          const R iK = arcp(K), isK = arcp(sK);
+          c0 = R(1) + R(0.5) * nu0 * iK;
+          c1 = R(0.5) * (R)a.CF * isK;
+          vmag = asqrt(vx * vx + vy * vy + vz * vz);
+          rr = asqrt(c0 * c0 + R(4) * c1 * vmag);
           D = c0 + rr;
-          const R s = R(2) / D;
+          const R s = R(2) * arcp(D);
           ux = vx * s;
           uy = vy * s;
           uz = vz * s;
-          dk = nu0 / K;
-          fk = (R)a.CF / sK;
-          umag = sqrt(ux * ux + uy * uy + uz * uz);
+          dk = nu0 * iK;
+          fk = (R)a.CF * isK;
+          umag = asqrt(ux * ux + uy * uy + uz * uz);
           const R cc = rho * (dk + fk * umag);
           Fx -= cc * ux;
           Fy -= cc * uy;
@@ -323,8 +352,8 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           pxy = pxy * i36 - rho * ux * uy;
           pxz = pxz * i36 - rho * ux * uz;
           pyz = pyz * i36 - rho * uy * uz;
-          Qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-          sroot = sqrt(tau0 * tau0 + (R)a.les_c * Qn * inv_rho);
+          Qn = asqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+          sroot = asqrt(tau0 * tau0 + (R)a.les_c * Qn * inv_rho);
           tau = R(0.5) * (tau0 + sroot);
           const R tmax = (R)a.tau_max;
           if (tau < tmax) {
@@ -333,7 +362,7 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
             tau = tmax;
           }
         }
-        const R om = R(1) / tau;
+        const R om = arcp(tau);
         const R cf = R(1) - R(0.5) * om;
         // ---- 3. adjoint of the collision, from the lambda moments ----
         //   sum lam feq / rho  = (M0 + rd M0 + rho (3 M1.u + 4.5 u.M2.u - 1.5 u.u M0)) / rho
@@ -374,8 +403,8 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           hy = gy;
           hz = gz;
           if (les_active) {
-            const R Lam = domega * (-om * om) * (R)a.les_c / (R(4) * sroot * rho);
-            const R iQn = R(1) / Qn;
+            const R Lam = domega * (-om * om) * (R)a.les_c * arcp(R(4) * sroot * rho);
+            const R iQn = arcp(Qn);
             const R trP3 = (pxx + pyy + pzz) * R(1.0 / 3.0);
             const R Pux = pxx * ux + pxy * uy + pxz * uz, Puy = pxy * ux + pyy * uy + pyz * uz,
                     Puz = pxz * ux + pyz * uy + pzz * uz;
@@ -397,18 +426,17 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           const R cdot = CF3[0] * ux + CF3[1] * uy + CF3[2] * uz;
           const R cc = dk + fk * umag;
           R gu[3];
-          for (int d = 0; d < 3; ++d) {
-            gu[d] = om * Au[d] + Bu[d] - rho * cc * CF3[d];
-            if (umag > R(0)) gu[d] -= rho * fk * cdot * u3[d] / umag;
-          }
+          const R fcu = umag > R(0) ? rho * fk * cdot * arcp(umag) : R(0);
+          for (int d = 0; d < 3; ++d) gu[d] = om * Au[d] + Bu[d] - rho * cc * CF3[d] - fcu * u3[d];
           const R grho = om * Arho - cc * cdot;
           // v-bar = (du/dv)^T gu,  du/dv = s I + (ds/d|v|) v v^T/|v|
-          const R s = R(2) / D;
-          const R dsdv = -R(4) * c1 / (rr * D * D);
+          const R iD = arcp(D);
+          const R s = R(2) * iD;
+          const R dsdv = -R(4) * c1 * arcp(rr) * iD * iD;
           const R vg = vx * gu[0] + vy * gu[1] + vz * gu[2];
           R vb[3] = {s * gu[0], s * gu[1], s * gu[2]};
           if (vmag > R(0)) {
-            const R t = dsdv * vg / vmag;
+            const R t = dsdv * vg * arcp(vmag);
             vb[0] += t * vx;
             vb[1] += t * vy;
             vb[2] += t * vz;
@@ -419,13 +447,13 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
           hz = vb[2] * inv_rho;
           C = grho - (hx * vx + hy * vy + hz * vz);
           // dJ/dK: via u (c0, c1) and via F (dk, fk)
-          const R K2 = K * K, K32 = K * sK;
-          const R dc0 = -nu0 / (R(2) * K2);
-          const R dc1 = -(R)a.CF / (R(4) * K32);
-          const R dr = (c0 * dc0 + R(2) * vmag * dc1) / rr;
-          const R dsdK = -R(2) / (D * D) * (dc0 + dr);
+          const R iK2 = arcp(K * K), iK32 = arcp(K * sK);
+          const R dc0 = R(-0.5) * nu0 * iK2;
+          const R dc1 = R(-0.25) * (R)a.CF * iK32;
+          const R dr = (c0 * dc0 + R(2) * vmag * dc1) * arcp(rr);
+          const R dsdK = R(-2) * iD * iD * (dc0 + dr);
           const R du_dK = vg * dsdK;  // gu . (v ds/dK)
-          const R ddk = -nu0 / K2, dfk = -(R)a.CF / (R(2) * K32);
+          const R ddk = -nu0 * iK2, dfk = R(-0.5) * (R)a.CF * iK32;
           const R dF = -rho * (ddk + dfk * umag) * cdot;
           if (A.grad_perm) A.grad_perm[cell] += (double)(du_dK + dF);
         }
@@ -441,9 +469,12 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
     const uint64_t group = 0xffffffffull << (threadIdx.x & 32);
     const bool store = SPARSE ? in_run : (inside && (active || a.chunks != nullptr || (any & group) != 0));
     if (store) {
+      // pairwise: e.P.e is shared by q and opp q, e.h changes sign
       R* __restrict__ mu_out = (R*)A.lam_out;
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
+      *AD::at(mu_out, own) = keep * lam[0] + C;
+      static_for<0, 9>([&](auto pc) {
+        constexpr int q = PAIR_A(decltype(pc)::value);
+        constexpr int o = OPP(q);
         R ePe = R(0);
         eacc<EX(q) * EX(q)>(ePe, Pxx);
         eacc<EY(q) * EY(q)>(ePe, Pyy);
@@ -451,23 +482,36 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
         eacc<EX(q) * EY(q)>(ePe, Pxy2);
         eacc<EX(q) * EZ(q)>(ePe, Pxz2);
         eacc<EY(q) * EZ(q)>(ePe, Pyz2);
-        *AD::at(mu_out + (AI)q * P, own) = keep * lam[q] + C + edot<EX(q), EY(q), EZ(q)>(hx, hy, hz) + L1 * ePe;
+        const R base = C + L1 * ePe;
+        const R eh = edot<EX(q), EY(q), EZ(q)>(hx, hy, hz);
+        *AD::at(mu_out + (AI)q * P, own) = keep * lam[q] + (base + eh);
+        *AD::at(mu_out + (AI)o * P, own) = keep * lam[o] + (base - eh);
       });
     }
   }
-  // block partials of dJ/du_in and J (FP64)
+  // block partials of dJ/du_in and J (FP64), added to the block's slot: the
+  // slots hold the sums over every step since pol_adjoint_reset and are
+  // folded once, in block order, by k_adjoint_final when collected
+  // (only inlet and outlet cells contribute: other waves write zeros and skip
+  // the FP64 shuffles)
   __shared__ double sh[BLOCK / 64][4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const bool contributes = part[0] != R(0) || part[1] != R(0) || part[2] != R(0) || part[3] != R(0);
+  if (__ballot(contributes)) {
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const double r = wave_sum((double)part[s]);
-    if (lane == 0) sh[wid][s] = r;
+    for (int s = 0; s < 4; ++s) {
+      const double r = wave_sum((double)part[s]);
+      if (lane == 0) sh[wid][s] = r;
+    }
+  } else if (lane == 0) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) sh[wid][s] = 0.0;
   }
   __syncthreads();
   if (threadIdx.x < 4) {
     double r = 0.0;
     for (int w = 0; w < BLOCK / 64; ++w) r += sh[w][threadIdx.x];
-    A.partial[(size_t)lblk * 4 + threadIdx.x] = r;
+    A.partial[(size_t)lblk * 4 + threadIdx.x] += r;  // this block's slot: summed over the steps
   }
 }
 
@@ -483,7 +527,7 @@ __global__ void __launch_bounds__(256) k_adjoint_final(const double* __restrict_
     if (lane == 0) sh[wid][s] = r;
   }
   __syncthreads();
-  if (threadIdx.x < 4) acc[threadIdx.x] += sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+  if (threadIdx.x < 4) acc[threadIdx.x] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
 
 template <typename S, bool SPARSE>

@@ -133,9 +133,19 @@ __device__ inline void set_equilibrium(R (&d)[Q], R rd, R rho, R ux, R uy, R uz)
 // a pair share their symmetric part and negate the antisymmetric one, and the
 // non-equilibrium stress needs only the pair sums (e_b e_b = e_a e_a), which
 // halves the arithmetic of a per-direction evaluation.
+// LES: nu_fd >= 0 is a given eddy viscosity (the reference's finite-difference
+// form, POL_LES_FD): tau = clamp(tau0 + 3 nu_fd, 0.55, 1.90) as the legacy
+// collision clamps it (legacy/lbm_solver.py:565-572); nu_fd < 0 selects the
+// local non-equilibrium-stress form below.
+template <typename R>
+__device__ __forceinline__ R tau_of_nu(R tau0, R nu_fd) {
+  const R t = tau0 + R(3) * nu_fd;
+  return t < R(0.55) ? R(0.55) : (t > R(1.90) ? R(1.90) : t);
+}
+
 template <typename R, bool FULL>
 __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous, bool les, R K, R fx, R fy, R fz,
-                                          R bx, R by, R bz, R B, const Phys<R>& P) {
+                                          R bx, R by, R bz, R B, const Phys<R>& P, R nu_fd = R(-1)) {
   CellOut<R> o;
   if (FULL && kind == POL_KIND_INLET) {
     o.rd = P.inlet_rho - R(1);
@@ -209,7 +219,9 @@ __device__ inline CellOut<R> collide_cell(R (&d)[Q], uint32_t kind, bool porous,
   const R base = rd - R(1.5) * rho * usq;  // shared part of every equilibrium
   const R d0eq = CW<R>(0) * base;
   R tau = P.tau0;
-  if (FULL && les && !porous) {
+  if (FULL && les && !porous && nu_fd >= R(0)) {
+    tau = tau_of_nu(P.tau0, nu_fd);
+  } else if (FULL && les && !porous) {
     // 36 x non-equilibrium stress Pi_ab = sum_q e_a e_b (f_q - feq_q); each pair
     // contributes e_a e_a^T (n_a + n_b) with n_a + n_b = (d_a + d_b) - 2 sym
     R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
@@ -302,7 +314,7 @@ __device__ inline float mix_fma(float k, uint32_t a, float c) {
 template <bool FULL, bool MIX = false>
 __device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, bool porous, bool les, float K,
                                                  float fx, float fy, float fz, float bx, float by, float bz, float B,
-                                                 const Phys<float>& P, const uint32_t (&h)[Q]) {
+                                                 const Phys<float>& P, const uint32_t (&h)[Q], float nu_fd = -1.0f) {
   using R = float;
   CellOut<R> o;
   if (FULL && kind == POL_KIND_INLET) {
@@ -380,7 +392,9 @@ __device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, b
   const R usq = ux * ux + uy * uy + uz * uz;
   const R base = rd - R(1.5) * rho * usq;
   R omega = P.omega0;
-  if (FULL && les && !porous) {
+  if (FULL && les && !porous && nu_fd >= R(0)) {
+    omega = __builtin_amdgcn_rcpf(tau_of_nu(P.tau0, nu_fd));
+  } else if (FULL && les && !porous) {
     R sxx = R(0), syy = R(0), szz = R(0), sxy = R(0), sxz = R(0), syz = R(0);
     static_for<0, 9>([&](auto pc) {
       constexpr int p = decltype(pc)::value;

@@ -25,6 +25,7 @@ struct StepArgs {
   const void* force;  // real [3][N]
   const void* perm;   // real [N]
   const void* pdrag;  // real [4][N] particle-drag field (sum w beta v, sum w beta)
+  const void* nu_sgs; // real [N] finite-difference eddy viscosity (POL_LES_FD), or null
   void* rho;          // real [N]
   void* u;            // real [3][N]
   double* partial;    // [nblk][POL_NRED]
@@ -47,6 +48,19 @@ struct StepArgs {
   double inlet_rho, inlet_u[3], outlet_rho;
   PhysF pf;
 };
+
+// Finite-difference eddy viscosity of the reference (les_fd.hip).
+struct LesFdArgs {
+  const void* u;          // real [3][N]
+  const uint32_t* flags;  // cell flags or null (LES everywhere)
+  const void* phase;      // real [N] or null (phi = 1)
+  void* nu;               // real [N] out
+  uint32_t nx, ny, nz, nxy, N;
+  FastDiv dnx, dny;
+  float cs2;              // (C_s * 1)^2, rounded once from double
+  float nu_max;
+};
+hipError_t launch_les_fd(bool dbl, const LesFdArgs& a, hipStream_t s);
 
 // Launch one pass.  store_f: write post-collision populations (a step);
 // false = moments only (macro/reduce of the current state).

@@ -152,10 +152,15 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     const bool les = FULL && (a.options & POL_LES) && !(fl & POL_FLAG_NO_LES);
     // moments-only passes use the same velocity definition the collision uses
     const bool les_on = STORE && les;
+    // the reference's finite-difference eddy viscosity, formed beforehand
+    R nu_fd = R(-1);
+    if constexpr (FULL && STORE) {
+      if (les_on && (a.options & POL_LES_FD)) nu_fd = ((const R*)a.nu_sgs)[cell];
+    }
     if constexpr (sizeof(S) == 2)
-      o = collide_cell_lp<FULL, MIX>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P, h);
+      o = collide_cell_lp<FULL, MIX>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P, h, nu_fd);
     else
-      o = collide_cell<R, FULL>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P);
+      o = collide_cell<R, FULL>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P, nu_fd);
   }
   if constexpr (STORE) {
     // Solid lanes next to fluid ones store zero deviations (f = w; no gather

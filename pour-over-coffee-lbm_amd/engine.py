@@ -57,6 +57,7 @@ class SolverConfig:
     force_field: bool = False
     perm_field: bool = False
     particle_field: bool = False     # implicit particle-drag field (two-way coupling)
+    les_fd: bool = False             # with les: the reference's finite-difference eddy viscosity
     device: int = 0
 
     @property
@@ -87,6 +88,7 @@ class SolverConfig:
         o |= L.POL_PERM_FIELD if self.perm_field else 0
         o |= L.POL_GRAVITY if self.gravity is not None else 0
         o |= L.POL_PARTICLE_FIELD if self.particle_field else 0
+        o |= L.POL_LES_FD if self.les_fd else 0
         return o
 
     def desc(self, use_flags: bool) -> L.PolDesc:
@@ -150,6 +152,10 @@ class HipContext:
         self.force = torch.zeros((3,) + shape, dtype=rd, device=self.device) if cfg.force_field else None
         self.perm = torch.full(shape, cfg.darcy_K, dtype=rd, device=self.device) if cfg.perm_field else None
         self.pdrag = torch.zeros((4,) + shape, dtype=rd, device=self.device) if cfg.particle_field else None
+        # finite-difference LES: the eddy viscosity and the optional phase field
+        # of its interface suppression (None: phi = 1)
+        self.nu_sgs = torch.zeros(shape, dtype=rd, device=self.device) if cfg.les_fd else None
+        self.les_phase: Optional[torch.Tensor] = None
         self._ctx = C.c_void_p()
         self._use_flags = flags is not None
         st = self.lib.pol_create(C.byref(self._ctx), C.byref(cfg.desc(self._use_flags)))
@@ -159,6 +165,13 @@ class HipContext:
     # -- lifecycle -----------------------------------------------------------
     def _bind(self) -> None:
         L.check(self.lib.pol_bind(self._ctx, C.byref(self.buffers())), self._ctx, "pol_bind")
+
+    def _bind_keep_state(self) -> None:
+        """Re-bind the buffers (after one of the optional ones changed) keeping
+        the current-state index."""
+        cur = self.current
+        self._bind()
+        self.set_current(cur)
 
     def buffers(self) -> L.PolBuffers:
         """The pol_buffers of this context (f[0] / f[1] are the two ping-pong
@@ -182,6 +195,8 @@ class HipContext:
             b.n_chunks = self.chunks.numel()
             b.chunk_cells = self.chunk_cells
         b.particle_drag = self.pdrag.data_ptr() if self.pdrag is not None else None
+        b.nu_sgs = self.nu_sgs.data_ptr() if self.nu_sgs is not None else None
+        b.les_phase = self.les_phase.data_ptr() if self.les_phase is not None else None
         return b
 
     def _set_layout(self, flags: Optional[np.ndarray]) -> None:

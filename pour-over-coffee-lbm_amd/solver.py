@@ -156,6 +156,15 @@ class HipLBMSolver:
         self.thermal_coupling_enabled = False
         self._pending_coupling = None  # particle system coupled by update_particle_coupling, not yet stepped
         self._last_particles = None
+        # the reference's finite-difference LES (legacy/lbm_solver.py:93-111
+        # hands the solver's phase field to the model)
+        self.les_model = None
+        if config.les_fd:
+            from .les import HipLESTurbulenceModel
+
+            self.les_model = HipLESTurbulenceModel(self.ctx)
+            self.les_model.set_phase_field(self.phase)
+            self.les_nu_sgs = self.les_model.nu_sgs
 
     # ---- fields ------------------------------------------------------------------
     @property
@@ -454,6 +463,15 @@ class HipLBMSolver:
         self.thermal_coupling_enabled = False
         self._pending_coupling = None  # particle system coupled by update_particle_coupling, not yet stepped
         self._last_particles = None
+        # the reference's finite-difference LES (legacy/lbm_solver.py:93-111
+        # hands the solver's phase field to the model)
+        self.les_model = None
+        if config.les_fd:
+            from .les import HipLESTurbulenceModel
+
+            self.les_model = HipLESTurbulenceModel(self.ctx)
+            self.les_model.set_phase_field(self.phase)
+            self.les_nu_sgs = self.les_model.nu_sgs
 
     # ---- fields ------------------------------------------------------------------
     @property

@@ -101,7 +101,7 @@ int main(void) {
   S(pol_buffers) F(pol_buffers, f) F(pol_buffers, cell_flags) F(pol_buffers, force) F(pol_buffers, perm)
   F(pol_buffers, rho) F(pol_buffers, u) F(pol_buffers, active_chunks) F(pol_buffers, n_chunks)
   F(pol_buffers, chunk_cells) F(pol_buffers, particle_drag) F(pol_buffers, chunk_records)
-  F(pol_buffers, stored_cells) F(pol_buffers, layout_flags)
+  F(pol_buffers, stored_cells) F(pol_buffers, layout_flags) F(pol_buffers, nu_sgs) F(pol_buffers, les_phase)
   S(pol_particle_desc) F(pol_particle_desc, count) F(pol_particle_desc, drag_law) F(pol_particle_desc, rho_fluid)
   F(pol_particle_desc, nu_fluid) F(pol_particle_desc, drag_max_per_mass) F(pol_particle_desc, relax_alpha)
   F(pol_particle_desc, gravity) F(pol_particle_desc, dt) F(pol_particle_desc, max_disp)

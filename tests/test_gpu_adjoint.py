@@ -128,8 +128,9 @@ def test_autograd_function(hip):
 
 def test_config5_adjoint_128(hip):
     """128^3 scaled cup, 500 steps, J over the last 100, checkpoints every 50:
-    FP64 adjoint vs central FD (eps 1e-3) on 16 random filter cells and on u_in;
-    FP32 adjoint vs FP64 adjoint; reverse cost per forward step."""
+    FP64 adjoint vs central FD (eps 1e-3) on 16 random filter cells and on
+    u_in (1e-3 relative); FP32 adjoint vs FP64 adjoint (1e-3 relative L2 on
+    dJ/dK, 1e-3 on dJ/du_in); reverse cost per forward step."""
     p64, K = _problem(128, "f64", 500, 100, 50)
     J, gK, gu = p64.gradient()
     gK = gK.cpu().numpy()
@@ -139,12 +140,26 @@ def test_config5_adjoint_128(hip):
     rel = np.abs(ad - fd) / np.maximum(np.abs(fd), 1e-300)
     print("config5 J %.6e max rel err vs FD %.2e" % (J, rel.max()))
     assert rel.max() < 1e-3
+    # inlet velocity: central FD of the FP64 forward run (eps 1e-3 relative)
+    u0 = p64.ctx.config.inlet_u
+    h = 1e-3 * abs(u0[2])
+    p64.set_inlet_velocity((u0[0], u0[1], u0[2] + h))
+    jp = p64.forward()
+    p64.set_inlet_velocity((u0[0], u0[1], u0[2] - h))
+    jm = p64.forward()
+    p64.set_inlet_velocity(u0)
+    fdu = (jp - jm) / (2 * h)
+    print("config5 u_in grad adjoint %.6e fd %.6e" % (gu[2], fdu))
+    assert fdu != 0.0
+    assert abs(gu[2] - fdu) <= 1e-3 * abs(fdu)
     p32, _ = _problem(128, "f32", 500, 100, 50)
     J32, g32, u32 = p32.gradient()
     t_fwd = p32.forward_time()
     step_cost = p32.timings["reverse_s"] / t_fwd   # recompute + adjoint kernel per forward step
     por = p64.wl.geometry.porous
     e = rel_l2(g32.cpu().numpy()[por], gK[por])
-    print("config5 f32 vs f64 dJ/dK rel L2 %.2e, adjoint step / forward step %.2f, whole gradient / forward %.2f"
-          % (e, step_cost, (p32.timings["forward_s"] + p32.timings["reverse_s"]) / t_fwd))
-    assert e < 1e-2
+    eu = abs(u32[2] - gu[2]) / abs(gu[2])
+    print("config5 f32 vs f64 dJ/dK rel L2 %.2e, dJ/du_in rel %.2e, adjoint step / forward step %.2f, "
+          "whole gradient / forward %.2f" % (e, eu, step_cost, (p32.timings["forward_s"] + p32.timings["reverse_s"]) / t_fwd))
+    assert e < 1e-3
+    assert eu < 1e-3
