@@ -73,10 +73,34 @@ def bench_v60(precision, steps, warmup):
     red = ctx.macroscopic(macro=False, reduce=True)
     active = counts["fluid"] + counts["inlet"] + counts["outlet"]
     layout = "compact row runs" if ctx.compact else "dense"
+    so_s = time_stream_only(ctx, steps, warmup)
     del ctx
     torch.cuda.empty_cache()
     return {"event_s": ev_s, "wall_s": wall, "active_cells": active, "cells": 224**3, "red": red.tolist(),
-            "counts": counts, "layout": layout}
+            "counts": counts, "layout": layout, "stream_only_s": so_s}
+
+
+def time_stream_only(ctx, steps, warmup):
+    """Seconds per collisionless streaming pass (pol_stream_only) of the
+    same state and layout: the step's loads and stores without the collision,
+    the access pattern's bandwidth ceiling for the step kernel."""
+    import torch
+
+    ctx.stream_only(warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    ctx.stream_only(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / 1e3 / steps
+
+
+def pattern_ceiling(step_s, so_s, algo_bytes):
+    """The collisionless pass as the step's access-pattern ceiling."""
+    return {"kernel": "collisionless streaming pass (pol_stream_only), same layout and launch",
+            "kernel_ms": round(so_s * 1e3, 4), "achieved_GBs": round(algo_bytes / so_s / 1e9, 1),
+            "step_frac_of_ceiling": round(so_s / step_s, 3)}
 
 
 def bench_periodic(precision, steps, warmup, n=224):
@@ -774,7 +798,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counter_traffic(prec),
-                     "algorithmic_bytes_per_launch": algo_bytes, "kernel_ms": round(k_s * 1e3, 4)},
+                     "algorithmic_bytes_per_launch": algo_bytes, "kernel_ms": round(k_s * 1e3, 4),
+                     "pattern_ceiling": pattern_ceiling(k_s, r["stream_only_s"], algo_bytes)},
         "all_site_mlups": round(r["cells"] / k_s / 1e6, 1),
         "reference_published": {"mlups_224": REFERENCE_MLUPS_224, "hardware": "Apple M1/M2 Pro, Metal",
                                 "note": "self-inconsistent prose figure, all-site count"},
@@ -794,7 +819,9 @@ def main():
                                        "kernel_ms": round(s16 * 1e3, 4), "speedup_vs_f32": round(k_s / s16, 3),
                                        "achieved_GBs": round(76 * r16["active_cells"] / s16 / 1e9, 1),
                                        "frac": round(76 * r16["active_cells"] / s16 / 1e9 / HBM_PEAK_GBS, 4),
-                                       "layout": r16["layout"], "traffic": counter_traffic(p16)}
+                                       "layout": r16["layout"], "traffic": counter_traffic(p16),
+                                       "pattern_ceiling": pattern_ceiling(s16, r16["stream_only_s"],
+                                                                          76 * r16["active_cells"])}
         log("bench: coupled 224^3 f32 + 50k particles")
         out["coupled_224_f32_50k_particles"] = bench_coupled("f32", args.steps, args.warmup)
         log("bench: adjoint 128^3 f32")

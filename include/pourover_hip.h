@@ -222,6 +222,14 @@ int pol_step_ex(pol_ctx *ctx, int nsteps, uint32_t what, double *red_out, void *
  * the bound f); the context's current-buffer index is left unchanged.  Used
  * to record forward states for the adjoint's checkpoint recomputation. */
 int pol_step_from(pol_ctx *ctx, const void *fin, void *fout, uint32_t what, double *red_out, void *stream);
+/* nsteps collisionless streaming passes (f[next] = the populations a step
+ * would gather -- bounce-back links, open faces, the layout's stores -- with
+ * no collision and no moments), swapping the current buffer like pol_step.
+ * Free streaming of the populations (omega = 0), and the step's memory access
+ * pattern without its arithmetic: timing it against pol_step gives the
+ * layout's bandwidth ceiling for the step.  No counterpart in the reference
+ * (its streaming always follows a collision). */
+int pol_stream_only(pol_ctx *ctx, int nsteps, void *stream);
 /* Moments of the current state (the gather the next step would collide):
  * rho/u buffers and/or FP64 reductions.  Replaces
  * compute_macroscopic_quantities (compute_backends.py:393-404,

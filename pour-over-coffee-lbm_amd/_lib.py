@@ -198,6 +198,7 @@ SIGNATURES = {
     "pol_step": (C.c_int, [_P, C.c_int, _P]),
     "pol_step_ex": (C.c_int, [_P, C.c_int, C.c_uint32, _P, _P]),
     "pol_step_from": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P]),
+    "pol_stream_only": (C.c_int, [_P, C.c_int, _P]),
     "pol_macroscopic": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "pol_reduce": (C.c_int, [_P, _P, _P]),
     "pol_reduce_noalloc": (C.c_int, [C.POINTER(PolDesc), C.POINTER(PolBuffers), _P, _P]),

@@ -406,6 +406,19 @@ int pol_step_ex(pol_ctx* c, int nsteps, uint32_t what, double* red_out, void* st
 
 int pol_step(pol_ctx* c, int nsteps, void* stream) { return pol_step_ex(c, nsteps, 0u, nullptr, stream); }
 
+int pol_stream_only(pol_ctx* c, int nsteps, void* stream) {
+  if (!c || nsteps < 0) return POL_ERR_INVALID;
+  if (!c->bound) return fail(c, POL_ERR_STATE, "pol_stream_only: buffers not bound");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(c->d.device);
+  for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
+    const StepArgs a = make_args(c, c->b.f[c->cur], c->b.f[1 - c->cur], 0u);
+    e = launch_stream_only(c->d.storage, a, c->wgrid, s);
+    if (e == hipSuccess) c->cur = 1 - c->cur;
+  }
+  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_stream_only");
+}
+
 int pol_step_from(pol_ctx* c, const void* fin, void* fout, uint32_t what, double* red_out, void* stream) {
   if (!c || !fin || !fout || fin == fout) return POL_ERR_INVALID;
   if (!c->bound) return fail(c, POL_ERR_STATE, "pol_step_from: buffers not bound");

@@ -65,6 +65,9 @@ hipError_t launch_les_fd(bool dbl, const LesFdArgs& a, hipStream_t s);
 // Launch one pass.  store_f: write post-collision populations (a step);
 // false = moments only (macro/reduce of the current state).
 hipError_t launch_stream_collide(int storage, const StepArgs& a, bool store_f, const Grid& g, hipStream_t s);
+// Collisionless streaming pass a.fin -> a.fout (the step's gather, bounce-back
+// links, open faces and stores, no collision, no moments).
+hipError_t launch_stream_only(int storage, const StepArgs& a, const Grid& g, hipStream_t s);
 // Fold per-block partials into out[POL_NRED] (device), deterministic order.
 hipError_t launch_reduce_final(const double* partial, uint32_t nblk, double* out, hipStream_t s);
 // f = feq(rho, u) (rho/u real-typed, may be null); a.records selects the

@@ -153,6 +153,45 @@ hipError_t launch_stream_collide(int storage, const StepArgs& a, bool store_f, c
   return hipErrorInvalidValue;
 }
 
+template <typename S, bool A32>
+static void launch_so(const StepArgs& a, bool flags, dim3 grid, hipStream_t s) {
+  dim3 block(BLOCK);
+  const bool nt = A32 && a.stream_store;
+  if (flags) {
+    if (nt)
+      hipLaunchKernelGGL((k_stream_collide<S, true, true, true, true, true, false, true, false>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_stream_collide<S, true, true, true, A32, false, false, true, false>), grid, block, 0, s, a);
+  } else if (!A32) {
+    hipLaunchKernelGGL((k_stream_collide<S, false, true, true, false, false, false, true, false>), grid, block, 0, s, a);
+  } else if (nt) {
+    hipLaunchKernelGGL((k_stream_collide<S, false, false, true, true, true, false, true, false>), grid, block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_stream_collide<S, false, false, true, true, false, false, true, false>), grid, block, 0, s, a);
+  }
+}
+
+template <typename S>
+static hipError_t dispatch_so(const StepArgs& a, const Grid& g, hipStream_t s) {
+  const dim3 grid(launch_blocks(a.order, g));
+  if (fits_addr32(Q, a.P, sizeof(S)))
+    launch_so<S, true>(a, a.flags != nullptr, grid, s);
+  else
+    launch_so<S, false>(a, a.flags != nullptr, grid, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_only(int storage, const StepArgs& a, const Grid& g, hipStream_t s) {
+  if (a.records) return launch_stream_only_sparse(storage, a, g, s);
+  switch (storage) {
+    case POL_F32: return dispatch_so<float>(a, g, s);
+    case POL_F16: return dispatch_so<_Float16>(a, g, s);
+    case POL_BF16: return dispatch_so<bf16_t>(a, g, s);
+    case POL_F64: return dispatch_so<double>(a, g, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_reduce_final(const double* partial, uint32_t nblk, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partial, nblk, out);
   return hipGetLastError();

@@ -43,6 +43,39 @@ static hipError_t dispatch_sparse(const StepArgs& a, bool store_f, const Grid& g
   return hipGetLastError();
 }
 
+template <typename S, bool A32, bool XF>
+static void launch_sparse_stream_only(const StepArgs& a, dim3 grid, hipStream_t s) {
+  if (a.stream_store)
+    hipLaunchKernelGGL((k_stream_collide<S, true, true, true, A32, true, true, XF, false>), grid, dim3(BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_stream_collide<S, true, true, true, A32, false, true, XF, false>), grid, dim3(BLOCK), 0, s, a);
+}
+
+template <typename S>
+static hipError_t dispatch_sparse_stream_only(const StepArgs& a, const Grid& g, hipStream_t s) {
+  const bool a32 = fits_addr32(Q, a.P, sizeof(S));
+  const dim3 grid(launch_blocks(a.order, g));
+  if (a32 && a.xface)
+    launch_sparse_stream_only<S, true, true>(a, grid, s);
+  else if (a32)
+    launch_sparse_stream_only<S, true, false>(a, grid, s);
+  else if (a.xface)
+    launch_sparse_stream_only<S, false, true>(a, grid, s);
+  else
+    launch_sparse_stream_only<S, false, false>(a, grid, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_only_sparse(int storage, const StepArgs& a, const Grid& g, hipStream_t s) {
+  switch (storage) {
+    case POL_F32: return dispatch_sparse_stream_only<float>(a, g, s);
+    case POL_F16: return dispatch_sparse_stream_only<_Float16>(a, g, s);
+    case POL_BF16: return dispatch_sparse_stream_only<bf16_t>(a, g, s);
+    case POL_F64: return dispatch_sparse_stream_only<double>(a, g, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_stream_collide_sparse(int storage, const StepArgs& a, bool store_f, const Grid& g, hipStream_t s) {
   switch (storage) {
     case POL_F32: return dispatch_sparse<float>(a, store_f, g, s);

@@ -48,7 +48,11 @@ __device__ inline Phys<R> make_phys(const StepArgs& a) {
   return P;
 }
 
-template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32, bool NT = false, bool SPARSE = false, bool XF = true>
+// COLLIDE = false: collisionless streaming (f_out = the gathered populations,
+// bounce-back links and open faces as in a step), the step's memory access
+// pattern with none of its arithmetic.
+template <typename S, bool FLAGS, bool FULL, bool STORE, bool A32, bool NT = false, bool SPARSE = false, bool XF = true,
+          bool COLLIDE = true>
 __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   static_assert(!SPARSE || (FLAGS && FULL), "the compact layout runs the general kernel");
   using C = Codec<S>;
@@ -87,10 +91,14 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   R f[Q];  // scaled deviations 36 (f - w)
   // FP16 compact launches keep the loaded words and let the collision widen
   // them inside its first operations (collide_cell_lp<.., true>)
-  constexpr bool MIX = SPARSE && std::is_same_v<S, _Float16>;
+  constexpr bool MIX = SPARSE && COLLIDE && std::is_same_v<S, _Float16>;
   uint32_t h[Q];
   CellOut<R> o{R(0), R(0), R(0), R(0)};
   if (active) {
+    if constexpr (!COLLIDE) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = R(0);  // inlet lanes gather nothing
+    }
     if constexpr (SPARSE) {
       if (kind != POL_KIND_INLET) {
         if constexpr (MIX) {
@@ -157,7 +165,9 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     if constexpr (FULL && STORE) {
       if (les_on && (a.options & POL_LES_FD)) nu_fd = ((const R*)a.nu_sgs)[cell];
     }
-    if constexpr (sizeof(S) == 2)
+    if constexpr (!COLLIDE)
+      ;
+    else if constexpr (sizeof(S) == 2)
       o = collide_cell_lp<FULL, MIX>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P, h, nu_fd);
     else
       o = collide_cell<R, FULL>(f, kind, porous, les_on, K, fx, fy, fz, bx, by, bz, B, P, nu_fd);
@@ -230,5 +240,6 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
 
 // compact-layout launches (lbm_step_sparse.hip)
 hipError_t launch_stream_collide_sparse(int storage, const StepArgs& a, bool store_f, const Grid& g, hipStream_t s);
+hipError_t launch_stream_only_sparse(int storage, const StepArgs& a, const Grid& g, hipStream_t s);
 
 }  // namespace pol

@@ -347,6 +347,11 @@ class HipContext:
         L.check(st, self._ctx, "pol_step")
         return red
 
+    def stream_only(self, n: int = 1) -> None:
+        """n collisionless streaming passes (pol_stream_only): the step's gather
+        and stores without its collision, the layout's bandwidth ceiling."""
+        L.check(self.lib.pol_stream_only(self._ctx, int(n), L.stream_handle()), self._ctx, "pol_stream_only")
+
     def step_from(self, fin: torch.Tensor, fout: torch.Tensor, what: int = 0) -> Optional[np.ndarray]:
         red = np.zeros(L.POL_NRED, dtype=np.float64) if what & L.POL_WANT_REDUCE else None
         st = self.lib.pol_step_from(self._ctx, L.ptr(fin), L.ptr(fout), int(what),
