@@ -185,6 +185,20 @@ def _omp_threads(n):
         return False
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(budget_s=10.0, single_budget_s=6.0):
     """The C/OpenMP restatement of the reference CPU backend
     (cpu_backend.py:79-237: BGK collide + pull stream, FP32, SoA) on the
@@ -222,6 +236,7 @@ def cpu_baseline(budget_s=10.0, single_budget_s=6.0):
     _omp_threads(threads)
     n, el = run(budget_s, 40)
     out = {"value": round(active * n / el / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(),
            "sample": "%d steps of the 224^3 V60 mask, FP32, C/OpenMP restatement of the reference CPU backend "
                      "(collide + pull stream), %d threads; MLUPS over the %d non-solid cells" % (n, threads, active),
            "all_site_mlups": round(224**3 * n / el / 1e6, 2)}
@@ -357,10 +372,22 @@ def bench_diagnostics(reps=20, host=True):
     for _ in range(reps):
         D.field_statistics(ctx)
     call_ms = (time.perf_counter() - t0) / reps * 1e3
+    # device time per call from HIP events on the launch stream: the statistics
+    # pass and its fold (plus the 344-byte copy of the results)
+    dev_ms = 0.0
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        D.field_statistics(ctx)
+        e1.record()
+        torch.cuda.synchronize()
+        dev_ms += e0.elapsed_time(e1) / reps
     n = wl.config.cells
     algo = 20 * n  # rho 4 B + u 12 B + flags 4 B per cell
     out = {"grid": [224, 224, 224], "ms_per_call": round(call_ms, 4), "algorithmic_bytes": algo,
-           "achieved_GBs_wall": round(algo / call_ms / 1e6, 1), "frac_wall": round(algo / call_ms / 1e6 / HBM_PEAK_GBS, 4)}
+           "achieved_GBs_wall": round(algo / call_ms / 1e6, 1), "frac_wall": round(algo / call_ms / 1e6 / HBM_PEAK_GBS, 4),
+           "device_ms_per_call": round(dev_ms, 4), "achieved_GBs_device": round(algo / dev_ms / 1e6, 1),
+           "frac_device": round(algo / dev_ms / 1e6 / HBM_PEAK_GBS, 4)}
     if host:
         sys.path.insert(0, ROOT)
         from oracle import diagnostics as OD

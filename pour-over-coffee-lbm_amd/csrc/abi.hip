@@ -40,6 +40,7 @@ struct pol_ctx {
   double* stat_partial = nullptr;   // [nz * stat_tiles][POL_NSTAT]
   double* stat_zpart = nullptr;     // [nz * stat_tiles]
   double* stat_dev = nullptr;       // [POL_NSTAT + nz]
+  double* stat_host = nullptr;      // pinned copy of stat_dev (one transfer per call)
   // multiphase phase field
   bool mp = false;
   pol_multiphase_desc md{};
@@ -136,15 +137,18 @@ int pol_create(pol_ctx** out, const pol_desc* desc) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->red_dev, POL_NRED * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->adj_partial, (size_t)c->part_cap * 4 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->adj_acc, 4 * sizeof(double));
-  // statistics: up to 8 blocks per plane (1792 blocks at 224^3, ~24 cells per
-  // thread: the per-block reduction of the 43 statistics is amortised)
+  // statistics: up to 12 blocks per plane (2688 blocks at 224^3, each lane
+  // ~8 groups of four cells: the per-block reduction of the 43 statistics is
+  // amortised; 224^3 V60 kernel 0.062 ms against 0.069 / 0.067 ms with 8 / 24)
   const uint64_t nxy = (uint64_t)d.nx * d.ny;
-  c->stat_tiles = (uint32_t)std::min<uint64_t>((nxy + BLOCK - 1) / BLOCK, 8);
+  c->stat_tiles = (uint32_t)std::min<uint64_t>((nxy + BLOCK - 1) / BLOCK, 12);
   const size_t sblk = (size_t)c->stat_tiles * d.nz;
   if (e == hipSuccess) e = hipMalloc((void**)&c->stat_partial, sblk * POL_NSTAT * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->stat_zpart, sblk * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->stat_dev, ((size_t)POL_NSTAT + d.nz) * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->stat_host, ((size_t)POL_NSTAT + d.nz) * sizeof(double));
   if (e != hipSuccess) {
+    (void)hipHostFree(c->stat_host);
     (void)hipFree(c->partial);
     (void)hipFree(c->red_dev);
     (void)hipFree(c->adj_partial);
@@ -169,6 +173,7 @@ int pol_destroy(pol_ctx* c) {
   (void)hipFree(c->stat_partial);
   (void)hipFree(c->stat_zpart);
   (void)hipFree(c->stat_dev);
+  (void)hipHostFree(c->stat_host);
   delete c;
   return POL_OK;
 }
@@ -835,6 +840,7 @@ int pol_field_stats(pol_ctx* c, const pol_stats_opts* o, double* out, double* z_
   s.nz = d.nz;
   s.nxy = (uint32_t)d.nx * d.ny;
   s.N = c->N;
+  s.dnx = make_fastdiv(d.nx);
   s.tiles = c->stat_tiles;
   s.z_top = o->z_top;
   s.z_mid = o->z_mid;
@@ -847,11 +853,14 @@ int pol_field_stats(pol_ctx* c, const pol_stats_opts* o, double* out, double* z_
     const StepArgs a = make_args(c, c->b.f[c->cur], nullptr, 0u);
     e = launch_pop_nonfinite(d.storage, a, c->partial, c->stat_dev, c->wgrid, st);
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(out, c->stat_dev, POL_NSTAT * sizeof(double), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess && z_profile)
-    e = hipMemcpyAsync(z_profile, c->stat_dev + POL_NSTAT, (size_t)d.nz * sizeof(double), hipMemcpyDeviceToHost, st);
+  // one transfer of statistics and profile into pinned memory, then host copies
+  const size_t nstat = (size_t)POL_NSTAT + (z_profile ? d.nz : 0);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->stat_host, c->stat_dev, nstat * sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  return e == hipSuccess ? POL_OK : hip_fail(c, e, "pol_field_stats");
+  if (e != hipSuccess) return hip_fail(c, e, "pol_field_stats");
+  std::memcpy(out, c->stat_host, POL_NSTAT * sizeof(double));
+  if (z_profile) std::memcpy(z_profile, c->stat_host + POL_NSTAT, (size_t)d.nz * sizeof(double));
+  return POL_OK;
 }
 
 int pol_export_fields(pol_ctx* c, void* magnitude, void* pressure, void* stream) {
@@ -1012,9 +1021,12 @@ static int pour_args(pol_ctx* c, const pol_pour_desc* d, PourArgs& p, const char
   p.flow_rate = (float)d->flow_rate;
   p.dt = (float)d->dt;
   p.height = d->height;
-  // bounding box of |(i, j) - c| <= radius, zmin <= k <= height, one cell of margin
-  auto lo = [](double v) { return (int64_t)std::floor(v) - 1; };
-  auto hi = [](double v) { return (int64_t)std::ceil(v) + 1; };
+  // bounding box of |(i, j) - c| <= radius, zmin <= k <= height, one cell of
+  // margin; bounds far outside the grid are clamped before the integer
+  // conversion (a finite centre or radius of any size stays defined)
+  auto lim = [](double v) { return std::min(std::max(v, -1.0e9), 1.0e9); };
+  auto lo = [&](double v) { return (int64_t)std::floor(lim(v)) - 1; };
+  auto hi = [&](double v) { return (int64_t)std::ceil(lim(v)) + 1; };
   const int64_t x0 = std::max<int64_t>(0, lo((double)p.cx - p.radius));
   const int64_t x1 = std::min<int64_t>(c->d.nx - 1, hi((double)p.cx + p.radius));
   const int64_t y0 = std::max<int64_t>(0, lo((double)p.cy - p.radius));

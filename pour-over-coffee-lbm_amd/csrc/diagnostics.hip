@@ -13,9 +13,10 @@
 // 224^3.  Here one pass reads each field once on the device.
 //
 // Statistics kernel: a grid of (tiles, nz) blocks, block (t, z) looping over
-// plane z; per-lane FP64 accumulators, wave64 shuffles, one partial per block
-// per statistic, folded in block order by one block (deterministic).  The
-// per-plane density sums of the z profile fall out of the same partials.
+// a strip of plane z; per-lane FP64 accumulators (32-bit counters for the
+// counts) split between two wave pairs, wave64 shuffles, one partial per
+// block per statistic, folded in block order by one block (deterministic).
+// The per-plane density sums of the z profile fall out of the same partials.
 // |u| is formed in FP32 as sqrt((ux*ux + uy*uy) + uz*uz) with explicit
 // round-to-nearest operations (no contraction), the value the reference's
 // float32 numpy expressions give per element.
@@ -78,147 +79,317 @@ __host__ __device__ constexpr bool stat_is_count(int s) {
          s == POL_STAT_FLUID_U_INF || s == POL_STAT_INT_CELLS || s == POL_STAT_UMAG_ACTIVE;
 }
 
-template <typename R>
-struct CellIn {
-  R r, ux, uy, uz, pr, pux, puy, puz, ph;
-  uint32_t fl;
-};
-
-template <typename R>
-__device__ inline void load_cell(const StatsArgs& a, size_t cell, CellIn<R>& c) {
-  const R* rho = (const R*)a.rho;
-  const R* u = (const R*)a.u;
-  const size_t N = a.N;
-  c.r = rho[cell];
-  c.ux = u[cell];
-  c.uy = u[N + cell];
-  c.uz = u[2 * N + cell];
-  c.fl = a.flags ? a.flags[cell] : (uint32_t)(POL_KIND_FLUID << POL_KIND_SHIFT);
-  if (a.prev_rho) c.pr = ((const R*)a.prev_rho)[cell];
-  if (a.prev_u) {
-    const R* pu = (const R*)a.prev_u;
-    c.pux = pu[cell];
-    c.puy = pu[N + cell];
-    c.puz = pu[2 * N + cell];
-  }
-  if (a.phase) c.ph = ((const R*)a.phase)[cell];
+// The statistics are split between the two wave pairs of a block, which walk
+// the same cells of the block's strip of plane z: waves 0-1 (role A) keep
+// the statistics of rho and of the velocity components (the sums over every
+// cell and the non-solid density checks), waves 2-3 (role B) everything that
+// needs |u| (its sums, Mach and activity, the non-solid and interior |u|
+// statistics) and the temporal change.  Each lane then holds about half of
+// the FP64 accumulators (twice the resident waves of a single set), |u| is
+// formed once per cell, and each wave reduces only its own statistics; the
+// second read of a cell comes from the cache.  The three regional |u| sums
+// are the block's |u| sum placed by its (block-uniform) plane.  When a plane
+// holds a multiple of four cells, lanes read four consecutive cells per load
+// (16-byte loads of rho, u and the flags).
+enum StatRole { ROLE_A = 0, ROLE_B = 1, ROLE_REGION = 2, ROLE_NONE = 3 };
+__host__ __device__ constexpr int stat_role(int s) {
+  return (s == POL_STAT_SUM_UMAG_TOP || s == POL_STAT_SUM_UMAG_MID || s == POL_STAT_SUM_UMAG_BOT) ? ROLE_REGION
+         : (s == POL_STAT_FLUID_POP_NAN || s == POL_STAT_FLUID_POP_INF)                          ? ROLE_NONE
+         : (s == POL_STAT_CELLS || s == POL_STAT_SUM_DRHO || s == POL_STAT_SUM_DRHO2 || s == POL_STAT_MAX_ABS_DRHO ||
+            (s >= POL_STAT_MOM_X && s <= POL_STAT_MOM_Z) || (s >= POL_STAT_SUM_UX && s <= POL_STAT_SUM_UZ2) ||
+            s == POL_STAT_OUTLET_DOWN || (s >= POL_STAT_FLUID && s <= POL_STAT_FLUID_RHO_MAX))
+             ? ROLE_A
+             : ROLE_B;
 }
 
-template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_field_stats(const StatsArgs a) {
-  const uint32_t z = blockIdx.y;
-  double v[POL_NSTAT];
-  uint32_t cnt[POL_NSTAT];
+// Wave reduction of one statistic's lane values into sh[wid][S] (lane 0).
+template <int S>
+__device__ inline void wave_stat(double v, uint32_t cnt, double (*sh)[POL_NSTAT + 1], int wid, int lane) {
+  if constexpr (stat_is_count(S)) {
 #pragma unroll
-  for (int s = 0; s < POL_NSTAT; ++s) {
-    v[s] = stat_identity(s);
-    cnt[s] = 0u;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane == 0) sh[wid][S] = (double)cnt;
+  } else {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = combine_c<S>(v, __shfl_xor(v, o, 64));
+    if (lane == 0) sh[wid][S] = v;
   }
-  double zs = 0.0;
-  const float cs = (float)0.57735026918962573;  // float32(1 / sqrt(3)), the reference's c_s
-  const uint32_t stride = a.tiles * BLOCK;
-  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  CellIn<R> c{}, nxt{};
-  if (i < a.nxy) load_cell<R>(a, (size_t)z * a.nxy + i, c);
-  // software-pipelined: the next cell's loads are in flight while this one reduces
-  for (; i < a.nxy; i += stride) {
-    const uint32_t in = i + stride;
-    if (in < a.nxy) load_cell<R>(a, (size_t)z * a.nxy + in, nxt);
-    const uint32_t y = i / a.nx, x = i - y * a.nx;
-    const double rd = (double)c.r;
-    const double um = (double)umag_f32(c.ux, c.uy, c.uz);
-    cnt[POL_STAT_CELLS] += 1u;
+}
+
+// four consecutive values of a plane (p 16-byte aligned)
+__device__ inline void load4(const float* p, float (&v)[4]) {
+  const float4 t = *(const float4*)p;
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+__device__ inline void load4(const double* p, double (&v)[4]) {
+  const double2 t0 = *(const double2*)p, t1 = *((const double2*)p + 1);
+  v[0] = t0.x;
+  v[1] = t0.y;
+  v[2] = t1.x;
+  v[3] = t1.y;
+}
+__device__ inline void load4(const uint32_t* p, uint32_t (&v)[4]) {
+  const uint4 t = *(const uint4*)p;
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+
+// Mach = |u| / c_s in FP32, correctly rounded, by the method for a divisor
+// known in advance (Brisebarre, Muller and Raina, IEEE Trans. Computers 53(8),
+// 2004): q = fma(x, zh, x * zl) with zh = RN(1 / c_s), zl = RN(1 / c_s - zh).
+// For c_s = float32(1 / sqrt(3)) this equals x / c_s for every x whose
+// product x * zl is a normal number (x >= 2^-101, checked exhaustively over
+// a binade on the host); below that both quotients are < 1e-33, under
+// every Mach threshold.  x = inf would give inf - inf, so it keeps x.
+__device__ inline float mach_of(float x) {
+#pragma clang fp contract(off)
+  constexpr float zh = 0x1.bb67bp+0f;      // RN(1 / c_s)
+  constexpr float zl = -0x1.e9ecd6p-25f;   // RN(1 / c_s - zh)
+  const float q = __builtin_fmaf(x, zh, x * zl);
+  return __builtin_isinf(x) ? x : q;
+}
+
+// Role A: rho and the velocity components.
+struct StatsA {
+  double s_d = 0.0, s_d2 = 0.0, mx_d = -1.0e300, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, s_ux = 0.0, s_uy = 0.0,
+         s_uz = 0.0, s_ux2 = 0.0, s_uy2 = 0.0, s_uz2 = 0.0, down = 0.0, zs = 0.0, r_min = 1.0e300, r_max = -1.0e300;
+  uint32_t cells = 0, fluid = 0, r_nan = 0, r_inf = 0;
+  template <typename R>
+  __device__ void add(R r, R ux, R uy, R uz, uint32_t fl, bool outlet) {
+    const double rd = (double)r;
+    const double dux = (double)ux, duy = (double)uy, duz = (double)uz;
+    cells += 1u;
     const double d = rd - 1.0;
-    v[POL_STAT_SUM_DRHO] += d;
-    v[POL_STAT_SUM_DRHO2] += d * d;
-    v[POL_STAT_MAX_ABS_DRHO] = nanmax(v[POL_STAT_MAX_ABS_DRHO], fabs(d));
-    v[POL_STAT_MOM_X] += rd * (double)c.ux;
-    v[POL_STAT_MOM_Y] += rd * (double)c.uy;
-    v[POL_STAT_MOM_Z] += rd * (double)c.uz;
-    v[POL_STAT_SUM_UMAG] += um;
-    v[POL_STAT_SUM_UMAG2] += um * um;
-    v[POL_STAT_MAX_UMAG] = nanmax(v[POL_STAT_MAX_UMAG], um);
-    const float mach = (float)um / cs;
+    s_d += d;
+    s_d2 += d * d;
+    mx_d = nanmax(mx_d, fabs(d));
+    mom_x += rd * dux;
+    mom_y += rd * duy;
+    mom_z += rd * duz;
+    s_ux += dux;
+    s_uy += duy;
+    s_uz += duz;
+    s_ux2 += dux * dux;
+    s_uy2 += duy * duy;
+    s_uz2 += duz * duz;
+    if (outlet) down += fmax(-duz, 0.0);
+    zs += rd;
+    if (((fl >> POL_KIND_SHIFT) & POL_KIND_MASK) != POL_KIND_SOLID) {
+      fluid += 1u;
+      if (isnan(rd)) r_nan += 1u;
+      else if (isinf(rd)) r_inf += 1u;
+      else {
+        r_min = fmin(r_min, rd);
+        r_max = fmax(r_max, rd);
+      }
+    }
+  }
+  __device__ void reduce(double (*sh)[POL_NSTAT + 1], int wid, int lane) {
+    wave_stat<POL_STAT_CELLS>(0.0, cells, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_DRHO>(s_d, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_DRHO2>(s_d2, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_MAX_ABS_DRHO>(mx_d, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_MOM_X>(mom_x, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_MOM_Y>(mom_y, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_MOM_Z>(mom_z, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UX>(s_ux, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UY>(s_uy, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UZ>(s_uz, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UX2>(s_ux2, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UY2>(s_uy2, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UZ2>(s_uz2, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_OUTLET_DOWN>(down, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID>(0.0, fluid, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_RHO_NAN>(0.0, r_nan, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_RHO_INF>(0.0, r_inf, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_RHO_MIN>(r_min, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_RHO_MAX>(r_max, 0u, sh, wid, lane);
+    const double z = wave_sum(zs);
+    if (lane == 0) sh[wid][POL_NSTAT] = z;
+  }
+};
+
+// Role B: |u| and what derives from it, and the temporal change.
+struct StatsB {
+  double s_um = 0.0, s_um2 = 0.0, mx_um = -1.0e300, s_mach = 0.0, s_uma = 0.0, p_r2 = 0.0, p_u2 = 0.0, f_umax = 0.0,
+         i_water = 0.0, i_air = 0.0, i_umax = -1.0e300, i_umin = 1.0e300, i_usum = 0.0;
+  uint32_t active = 0, mach_gt = 0, um_active = 0, u_nan = 0, u_inf = 0, i_cells = 0;
+  // i: the cell's index within plane z (base = z nxy); zin: 1 <= z <= nz - 2
+  template <typename R>
+  __device__ void add(const StatsArgs& a, size_t base, uint32_t i, bool zin, R r, R ux, R uy, R uz, uint32_t fl) {
+    const double rd = (double)r;
+    const float umf = umag_f32(ux, uy, uz);
+    const double um = (double)umf;
+    s_um += um;
+    s_um2 += um * um;
+    mx_um = nanmax(mx_um, um);
+    const float mach = mach_of(umf);
     if (mach > 1e-10f) {
-      v[POL_STAT_SUM_MACH_ACTIVE] += (double)mach;
-      cnt[POL_STAT_ACTIVE] += 1u;
+      s_mach += (double)mach;
+      active += 1u;
     }
-    if (mach > 0.1f) cnt[POL_STAT_MACH_GT] += 1u;
-    if ((float)um > 1e-10f) {
-      v[POL_STAT_SUM_UMAG_ACTIVE] += um;
-      cnt[POL_STAT_UMAG_ACTIVE] += 1u;
+    if (mach > 0.1f) mach_gt += 1u;
+    if (umf > 1e-10f) {
+      s_uma += um;
+      um_active += 1u;
     }
-    v[POL_STAT_SUM_UX] += (double)c.ux;
-    v[POL_STAT_SUM_UY] += (double)c.uy;
-    v[POL_STAT_SUM_UZ] += (double)c.uz;
-    v[POL_STAT_SUM_UX2] += (double)c.ux * (double)c.ux;
-    v[POL_STAT_SUM_UY2] += (double)c.uy * (double)c.uy;
-    v[POL_STAT_SUM_UZ2] += (double)c.uz * (double)c.uz;
-    if ((int)z >= a.z_top) v[POL_STAT_SUM_UMAG_TOP] += um;
-    else if ((int)z >= a.z_mid) v[POL_STAT_SUM_UMAG_MID] += um;
-    else v[POL_STAT_SUM_UMAG_BOT] += um;
-    if ((int)z < a.z_out) v[POL_STAT_OUTLET_DOWN] += fmax(-(double)c.uz, 0.0);
     if (a.prev_rho) {
-      const double e = rd - (double)c.pr;
-      v[POL_STAT_DRHO_PREV2] += e * e;
+      const double e = rd - (double)((const R*)a.prev_rho)[base + i];
+      p_r2 += e * e;
     }
     if (a.prev_u) {
-      const double ex = (double)c.ux - (double)c.pux, ey = (double)c.uy - (double)c.puy,
-                   ez = (double)c.uz - (double)c.puz;
-      v[POL_STAT_DU_PREV2] += ex * ex + ey * ey + ez * ez;
+      const R* pu = (const R*)a.prev_u + base + i;
+      const double ex = (double)ux - (double)pu[0], ey = (double)uy - (double)pu[a.N],
+                   ez = (double)uz - (double)pu[2 * (size_t)a.N];
+      p_u2 += ex * ex + ey * ey + ez * ez;
     }
-    zs += rd;
-    const uint32_t kind = (c.fl >> POL_KIND_SHIFT) & POL_KIND_MASK;
-    if (kind != POL_KIND_SOLID) {
-      cnt[POL_STAT_FLUID] += 1u;
-      if (isnan(rd)) cnt[POL_STAT_FLUID_RHO_NAN] += 1u;
-      else if (isinf(rd)) cnt[POL_STAT_FLUID_RHO_INF] += 1u;
-      else {
-        v[POL_STAT_FLUID_RHO_MIN] = fmin(v[POL_STAT_FLUID_RHO_MIN], rd);
-        v[POL_STAT_FLUID_RHO_MAX] = fmax(v[POL_STAT_FLUID_RHO_MAX], rd);
-      }
-      if (isnan(um)) cnt[POL_STAT_FLUID_U_NAN] += 1u;
-      else if (isinf(um)) cnt[POL_STAT_FLUID_U_INF] += 1u;
-      else v[POL_STAT_FLUID_UMAX] = fmax(v[POL_STAT_FLUID_UMAX], um);
-      if (x >= 1 && x + 1 < a.nx && y >= 1 && y + 1 < a.ny && z >= 1 && z + 1 < a.nz) {
-        cnt[POL_STAT_INT_CELLS] += 1u;
-        const bool water = a.phase ? (c.ph > R(0.1)) : true;
-        if (water) v[POL_STAT_INT_WATER] += rd;
-        else v[POL_STAT_INT_AIR] += rd;
-        v[POL_STAT_INT_UMAX] = fmax(v[POL_STAT_INT_UMAX], um);
-        v[POL_STAT_INT_UMIN] = fmin(v[POL_STAT_INT_UMIN], um);
-        v[POL_STAT_INT_USUM] += um;
+    if (((fl >> POL_KIND_SHIFT) & POL_KIND_MASK) != POL_KIND_SOLID) {
+      if (isnan(um)) u_nan += 1u;
+      else if (isinf(um)) u_inf += 1u;
+      else f_umax = fmax(f_umax, um);
+      const uint32_t y = fdiv(i, a.dnx), x = i - y * a.nx;
+      if (zin && x >= 1 && x + 1 < a.nx && y >= 1 && y + 1 < a.ny) {
+        i_cells += 1u;
+        const bool water = a.phase ? (((const R*)a.phase)[base + i] > R(0.1)) : true;
+        // selects, not a branch on which accumulator (adding +0 leaves a sum
+        // that starts at +0 unchanged)
+        i_water += water ? rd : 0.0;
+        i_air += water ? 0.0 : rd;
+        i_umax = fmax(i_umax, um);
+        i_umin = fmin(i_umin, um);
+        i_usum += um;
       }
     }
-    c = nxt;
   }
-  static_for<0, POL_NSTAT>([&](auto sc) {
-    constexpr int S = decltype(sc)::value;
-    if constexpr (stat_is_count(S)) v[S] = (double)cnt[S];
-  });
-  // compile-time slot indices keep the accumulators in registers
+  __device__ void reduce(double (*sh)[POL_NSTAT + 1], int wid, int lane) {
+    wave_stat<POL_STAT_SUM_UMAG>(s_um, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UMAG2>(s_um2, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_MAX_UMAG>(mx_um, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_MACH_ACTIVE>(s_mach, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_ACTIVE>(0.0, active, sh, wid, lane);
+    wave_stat<POL_STAT_MACH_GT>(0.0, mach_gt, sh, wid, lane);
+    wave_stat<POL_STAT_DRHO_PREV2>(p_r2, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_DU_PREV2>(p_u2, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_U_NAN>(0.0, u_nan, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_U_INF>(0.0, u_inf, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_UMAX>(f_umax, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_INT_CELLS>(0.0, i_cells, sh, wid, lane);
+    wave_stat<POL_STAT_INT_WATER>(i_water, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_INT_AIR>(i_air, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_INT_UMAX>(i_umax, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_INT_UMIN>(i_umin, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_INT_USUM>(i_usum, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_SUM_UMAG_ACTIVE>(s_uma, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_UMAG_ACTIVE>(0.0, um_active, sh, wid, lane);
+  }
+};
+
+// rho, u and the flags of four consecutive cells (group j of a plane)
+template <typename R>
+struct Group4 {
+  R r[4], x[4], y[4], w[4];
+  uint32_t f[4];
+  __device__ void load(const R* rho, const R* ux, const R* uy, const R* uz, const uint32_t* flp, uint32_t j,
+                       uint32_t fluid_word) {
+    load4(rho + 4 * (size_t)j, r);
+    load4(ux + 4 * (size_t)j, x);
+    load4(uy + 4 * (size_t)j, y);
+    load4(uz + 4 * (size_t)j, w);
+    if (flp) load4(flp + 4 * (size_t)j, f);
+    else f[0] = f[1] = f[2] = f[3] = fluid_word;
+  }
+};
+
+// Role loops over plane z: lane t of the pair takes cells (VEC: 4-cell
+// groups) t, t + stride, ... of the plane.
+template <typename R, bool VEC>
+__device__ inline void stats_role_a(const StatsArgs& a, uint32_t z, uint32_t t0, uint32_t stride,
+                                    double (*sh)[POL_NSTAT + 1], int wid, int lane) {
+  StatsA acc;
+  const bool outlet = (int)z < a.z_out;
+  const size_t base = (size_t)z * a.nxy;
+  const R* rho = (const R*)a.rho + base;
+  const R* ux_p = (const R*)a.u + base;
+  const R* uy_p = ux_p + a.N;
+  const R* uz_p = uy_p + a.N;
+  const uint32_t* flp = a.flags ? a.flags + base : nullptr;
+  const uint32_t fluid_word = (uint32_t)(POL_KIND_FLUID << POL_KIND_SHIFT);
+  if constexpr (VEC) {
+    for (uint32_t j = t0; j < a.nxy / 4; j += stride) {
+      Group4<R> g;
+      g.load(rho, ux_p, uy_p, uz_p, flp, j, fluid_word);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc.add<R>(g.r[k], g.x[k], g.y[k], g.w[k], g.f[k], outlet);
+    }
+  } else {
+    for (uint32_t i = t0; i < a.nxy; i += stride)
+      acc.add<R>(rho[i], ux_p[i], uy_p[i], uz_p[i], flp ? flp[i] : fluid_word, outlet);
+  }
+  acc.reduce(sh, wid, lane);
+}
+
+template <typename R, bool VEC>
+__device__ inline void stats_role_b(const StatsArgs& a, uint32_t z, uint32_t t0, uint32_t stride,
+                                    double (*sh)[POL_NSTAT + 1], int wid, int lane) {
+  StatsB acc;
+  const size_t base = (size_t)z * a.nxy;
+  const R* rho = (const R*)a.rho + base;
+  const R* ux_p = (const R*)a.u + base;
+  const R* uy_p = ux_p + a.N;
+  const R* uz_p = uy_p + a.N;
+  const uint32_t* flp = a.flags ? a.flags + base : nullptr;
+  const uint32_t fluid_word = (uint32_t)(POL_KIND_FLUID << POL_KIND_SHIFT);
+  const bool zin = z >= 1 && z + 1 < a.nz;
+  if constexpr (VEC) {
+    for (uint32_t j = t0; j < a.nxy / 4; j += stride) {
+      Group4<R> g;
+      g.load(rho, ux_p, uy_p, uz_p, flp, j, fluid_word);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc.add<R>(a, base, 4 * j + k, zin, g.r[k], g.x[k], g.y[k], g.w[k], g.f[k]);
+    }
+  } else {
+    for (uint32_t i = t0; i < a.nxy; i += stride)
+      acc.add<R>(a, base, i, zin, rho[i], ux_p[i], uy_p[i], uz_p[i], flp ? flp[i] : fluid_word);
+  }
+  acc.reduce(sh, wid, lane);
+}
+
+template <typename R, bool VEC>
+__global__ void __launch_bounds__(BLOCK) k_field_stats(const StatsArgs a) {
+  static_assert(BLOCK == 256, "two wave pairs per block");
+  const uint32_t z = blockIdx.y;
   __shared__ double sh[BLOCK / 64][POL_NSTAT + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  static_for<0, POL_NSTAT>([&](auto sc) {
-    constexpr int S = decltype(sc)::value;
-    double r = v[S];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) r = combine_c<S>(r, __shfl_xor(r, o, 64));
-    if (lane == 0) sh[wid][S] = r;
-  });
-  zs = wave_sum(zs);
-  if (lane == 0) sh[wid][POL_NSTAT] = zs;
+  const uint32_t t0 = blockIdx.x * (BLOCK / 2) + (threadIdx.x & (BLOCK / 2 - 1));
+  const uint32_t stride = a.tiles * (BLOCK / 2);
+  if (wid < 2) stats_role_a<R, VEC>(a, z, t0, stride, sh, wid, lane);
+  else stats_role_b<R, VEC>(a, z, t0, stride, sh, wid, lane);
   __syncthreads();
   const size_t blk = (size_t)z * a.tiles + blockIdx.x;
   if (threadIdx.x < POL_NSTAT) {
     const int s = threadIdx.x;
-    double r = sh[0][s];
-    for (int w = 1; w < BLOCK / 64; ++w) r = stat_combine(s, r, sh[w][s]);
+    double r;
+    switch (stat_role(s)) {
+      case ROLE_A: r = stat_combine(s, sh[0][s], sh[1][s]); break;
+      case ROLE_B: r = stat_combine(s, sh[2][s], sh[3][s]); break;
+      case ROLE_REGION: {
+        // the block's |u| sum, in the statistic of its plane's region
+        const int region = (int)z >= a.z_top ? POL_STAT_SUM_UMAG_TOP
+                           : (int)z >= a.z_mid ? POL_STAT_SUM_UMAG_MID
+                                               : POL_STAT_SUM_UMAG_BOT;
+        r = s == region ? sh[2][POL_STAT_SUM_UMAG] + sh[3][POL_STAT_SUM_UMAG] : 0.0;
+        break;
+      }
+      default: r = stat_identity(s);
+    }
     a.partial[blk * POL_NSTAT + s] = r;
   } else if (threadIdx.x == POL_NSTAT) {
-    double r = sh[0][POL_NSTAT];
-    for (int w = 1; w < BLOCK / 64; ++w) r += sh[w][POL_NSTAT];
-    a.zpart[blk] = r;
+    a.zpart[blk] = sh[0][POL_NSTAT] + sh[1][POL_NSTAT];
   }
 }
 
@@ -376,10 +547,17 @@ __global__ void __launch_bounds__(256) k_export_fields(const R* __restrict__ rho
 hipError_t launch_field_stats(bool real_double, const StatsArgs& s, double* out_dev, double* zout_dev,
                               hipStream_t st) {
   const dim3 grid(s.tiles, s.nz);
-  if (real_double)
-    hipLaunchKernelGGL(k_field_stats<double>, grid, dim3(BLOCK), 0, st, s);
+  // 16-byte loads need every plane (and so every u component) to start on a
+  // 4-cell boundary
+  const bool vec = s.nxy % 4 == 0;
+  if (real_double && vec)
+    hipLaunchKernelGGL((k_field_stats<double, true>), grid, dim3(BLOCK), 0, st, s);
+  else if (real_double)
+    hipLaunchKernelGGL((k_field_stats<double, false>), grid, dim3(BLOCK), 0, st, s);
+  else if (vec)
+    hipLaunchKernelGGL((k_field_stats<float, true>), grid, dim3(BLOCK), 0, st, s);
   else
-    hipLaunchKernelGGL(k_field_stats<float>, grid, dim3(BLOCK), 0, st, s);
+    hipLaunchKernelGGL((k_field_stats<float, false>), grid, dim3(BLOCK), 0, st, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_stats_final, dim3(POL_NSTAT + s.nz), dim3(256), 0, st, s.partial, s.zpart, s.tiles * s.nz,

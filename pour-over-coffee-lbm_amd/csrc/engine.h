@@ -85,6 +85,7 @@ struct StatsArgs {
   const void* prev_u;     // may be null
   const void* phase;      // may be null (every cell water)
   uint32_t nx, ny, nz, nxy, N;
+  FastDiv dnx;
   uint32_t tiles;         // blocks per z plane
   int32_t z_top, z_mid, z_out;
   double* partial;        // [nz * tiles][POL_NSTAT]

@@ -124,12 +124,13 @@ class PrecisePouringSystem:
         d.height = int(self.POUR_HEIGHT)
         return d
 
-    def apply_pouring_force(self, *args) -> None:
-        """precise_pouring.py:132-162: the last positional argument is dt (the
-        body force and solid mask are the solver context's)."""
+    def apply_pouring_force(self, *args, dt=None) -> None:
+        """precise_pouring.py:132-162, apply_pouring_force(body_force, solid, dt):
+        dt is the keyword or the last positional argument (default 1.0); the
+        body force and solid mask are the solver context's."""
         if self.pouring_active != 1:
             return
-        dt = float(args[-1]) if args else 1.0
+        dt = float(dt) if dt is not None else (float(args[-1]) if args else 1.0)
         self.pour_time = F(self.pour_time + F(dt))
         L.check(self.lib.pol_pour_force(self.ctx.handle, C.byref(self._desc(dt)), L.stream_handle()),
                 self.ctx.handle, "pol_pour_force")

@@ -146,3 +146,59 @@ def test_porous_cells_have_les_off(g48):
     assert por.any()
     assert ((g48.flags[por] & L.POL_FLAG_NO_LES) != 0).all()
     assert (_kind(g48.flags)[por] == L.POL_KIND_FLUID).all()
+
+
+@pytest.fixture(scope="module")
+def g224():
+    return build_v60()
+
+
+def test_cone_half_angle_is_30_degrees(g224):
+    """The cup wall measured from the 224^3 mask: in every plane between the
+    cone base and the top of the cone the open radius (largest centre
+    distance of a non-solid cell) grows linearly with z; the least-squares
+    slope is tan(half-angle), and the half-angle is 30 deg (the V60 workload:
+    30 deg cone, outlet hole radius 8 cells)."""
+    n = 224
+    kind = g224.kind
+    zb = g224.params.hole_length
+    yy, xx = np.mgrid[0:n, 0:n]
+    r = np.sqrt((xx - 111.5) ** 2 + (yy - 111.5) ** 2)
+    zs = np.arange(zb + 4, g224.cone_top_z - 4)
+    radii = np.array([r[kind[z] != L.POL_KIND_SOLID].max() for z in zs])
+    slope, icpt = np.polyfit(zs, radii, 1)
+    angle = np.degrees(np.arctan(slope))
+    assert abs(angle - 30.0) < 0.25, angle
+    # a cone, not a staircase or a curve: within one cell of the line everywhere
+    assert np.abs(radii - (slope * zs + icpt)).max() < 1.0
+    # and the line meets the outlet hole (radius 8) at the cone base
+    assert abs(slope * zb + icpt - 8.0) < 1.0
+    # above the cone top the wall is vertical (the cup's rim radius)
+    above = [r[kind[z] != L.POL_KIND_SOLID].max() for z in range(g224.cone_top_z + 1, n - 2)]
+    assert max(above) - min(above) < 1e-9
+
+
+def test_filter_layer_is_two_cells_thick(g224):
+    """The porous filter layer measured from the 224^3 mask: along radial
+    lines at several heights and along the axis through the base disk, each
+    crossing of the paper is two porous cells (the 2-cell filter layer)."""
+    n = 224
+    por = g224.porous
+    zb = g224.params.hole_length
+
+    def runs(line):
+        """lengths of the runs of True in a 1-D boolean array"""
+        d = np.diff(np.concatenate(([0], line.astype(np.int8), [0])))
+        return list(np.flatnonzero(d == -1) - np.flatnonzero(d == 1))
+
+    # radial lines along +x, -x, +y, -y through the axis (row 111: 0.5 cells off it)
+    for z in (zb + 20, zb + 60, zb + 100, zb + 140):
+        for line in (por[z, 111, 112:], por[z, 111, :112][::-1], por[z, 112:, 111], por[z, :112, 111][::-1]):
+            assert runs(line) == [2], (z, runs(line))
+    # the base disk closes the cone: two cells thick along z at the axis
+    for (y, x) in ((111, 111), (112, 112), (109, 113)):  # inside the disk (r < 5 at the base)
+        col = por[:, y, x]
+        assert runs(col) == [2], runs(col)
+        assert col[zb] and col[zb + 1]
+    # the layer is porous and has LES off; nothing else is
+    assert np.array_equal(por, (g224.flags & L.POL_FLAG_POROUS) != 0)
