@@ -108,8 +108,9 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
                                             });
           f[0] = (R)__builtin_bit_cast(_Float16, (uint16_t)h[0]);
         } else {
-          compact_sources<S, A32, XF>(fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl,
-                                      [&](auto qc, R v) { f[decltype(qc)::value] = v; });
+          compact_sources<S, A32, XF, false, COLLIDE && sizeof(S) == 2>(
+              fin, PS, rb, x, a.nx, (a.options & POL_PERIODIC_X) != 0, own, fl,
+              [&](auto qc, R v) { f[decltype(qc)::value] = v; });
         }
       }
     } else if (!FULL || kind != POL_KIND_INLET) {
