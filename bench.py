@@ -18,7 +18,7 @@ restatements on the host cores, bounded samples).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--precision P]
        [--coupled-only | --adjoint-only | --diagnostics-only | --thermal-only | --multiphase-only |
-        --forcing-only | --main-loop-only]
+        --forcing-only | --main-loop-only | --periodic-only]
        [--calibrate-copy]
 """
 from __future__ import annotations
@@ -749,6 +749,8 @@ def main():
                     help="only the 224^3 phase-field measurement (for profiling)")
     ap.add_argument("--forcing-only", action="store_true",
                     help="only the 224^3 body-force producer measurement (for profiling)")
+    ap.add_argument("--periodic-only", action="store_true",
+                    help="only the all-fluid periodic 224^3 box (dense layout) measurement (for profiling)")
     ap.add_argument("--main-loop-only", action="store_true",
                     help="only the 224^3 whole main-loop step measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -785,6 +787,16 @@ def main():
         log("bench: multiphase 224^3 f32")
         print(json.dumps({"multiphase_224_f32": bench_multiphase(reps=args.steps, warmup=args.warmup, cpu=False)}),
               flush=True)
+        return
+    if args.periodic_only:
+        log("bench: periodic 224^3 box %s" % prec)
+        p = bench_periodic(prec, args.steps, args.warmup)
+        ps = p["event_s"] / args.steps
+        nb = BYTES_PER_UPDATE[prec] * p["cells"]
+        print(json.dumps({"periodic_224_%s" % prec: {"mlups": round(p["cells"] / ps / 1e6, 1),
+                                                      "kernel_ms": round(ps * 1e3, 4),
+                                                      "achieved_GBs": round(nb / ps / 1e9, 1),
+                                                      "frac": round(nb / ps / 1e9 / HBM_PEAK_GBS, 4)}}), flush=True)
         return
     if args.main_loop_only:
         log("bench: main-loop step 224^3 f32")
