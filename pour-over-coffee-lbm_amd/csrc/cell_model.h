@@ -326,7 +326,7 @@ __device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, b
     return o;
   }
   R sp[9];  // pair sums d_a + d_b
-  R s36 = d[0], mx = R(0), my = R(0), mz = R(0);
+  R s36 = d[0], mx = R(-0.0), my = R(-0.0), mz = R(-0.0);  // -0: the additive identity, folded
   static_for<0, 9>([&](auto pc) {
     constexpr int p = decltype(pc)::value;
     constexpr int a = PAIR_A(p);
@@ -395,7 +395,7 @@ __device__ inline CellOut<float> collide_cell_lp(float (&d)[Q], uint32_t kind, b
   if (FULL && les && !porous && nu_fd >= R(0)) {
     omega = __builtin_amdgcn_rcpf(tau_of_nu(P.tau0, nu_fd));
   } else if (FULL && les && !porous) {
-    R sxx = R(0), syy = R(0), szz = R(0), sxy = R(0), sxz = R(0), syz = R(0);
+    R sxx = R(-0.0), syy = R(-0.0), szz = R(-0.0), sxy = R(-0.0), sxz = R(-0.0), syz = R(-0.0);
     static_for<0, 9>([&](auto pc) {
       constexpr int p = decltype(pc)::value;
       constexpr int a = PAIR_A(p);
