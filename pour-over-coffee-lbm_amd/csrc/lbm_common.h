@@ -71,6 +71,15 @@ struct Codec<double> {
   __device__ static inline void store_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
 };
 
+// The FP16 codec also stores two values at once: one packed conversion
+// (v_cvt_pk_f16_f32, round-to-nearest-even like the single one) and a
+// low-half and a high-half 16-bit store.  BF16 keeps single conversions: the
+// two-element form of its conversion (fptrunc of a float pair to a __bf16
+// pair) is compiled wrongly by this toolchain (the second element is
+// dropped and the first stored twice).
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+
 template <>
 struct Codec<_Float16> {
   using real = float;
@@ -78,6 +87,17 @@ struct Codec<_Float16> {
   __device__ static inline float load(const _Float16* p) { return (float)(*p); }
   __device__ static inline void store(_Float16* p, float v) { *p = (_Float16)v; }
   __device__ static inline void store_nt(_Float16* p, float v) { __builtin_nontemporal_store((_Float16)v, p); }
+  template <bool NT>
+  __device__ static inline void store2(_Float16* p0, _Float16* p1, float a, float b) {
+    const f16x2_t h = __builtin_convertvector((f32x2_t){a, b}, f16x2_t);
+    if constexpr (NT) {
+      __builtin_nontemporal_store(h.x, p0);
+      __builtin_nontemporal_store(h.y, p1);
+    } else {
+      *p0 = h.x;
+      *p1 = h.y;
+    }
+  }
 };
 
 __device__ inline float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }

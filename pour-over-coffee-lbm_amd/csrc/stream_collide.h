@@ -196,11 +196,21 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     }
     if (store) {
       // per-plane scalar base + the lane's one slot offset
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if constexpr (NT) C::store_nt(AD::at(fout + (AI)q * PS, own), f[q]);
-        else C::store(AD::at(fout + (AI)q * PS, own), f[q]);
-      });
+      if constexpr (std::is_same_v<S, _Float16>) {
+        // FP16 storage: planes in pairs, one packed conversion per pair
+        static_for<0, Q / 2>([&](auto pc) {
+          constexpr int q = 2 * decltype(pc)::value;
+          C::template store2<NT>(AD::at(fout + (AI)q * PS, own), AD::at(fout + (AI)(q + 1) * PS, own), f[q], f[q + 1]);
+        });
+        if constexpr (NT) C::store_nt(AD::at(fout + (AI)(Q - 1) * PS, own), f[Q - 1]);
+        else C::store(AD::at(fout + (AI)(Q - 1) * PS, own), f[Q - 1]);
+      } else {
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if constexpr (NT) C::store_nt(AD::at(fout + (AI)q * PS, own), f[q]);
+          else C::store(AD::at(fout + (AI)q * PS, own), f[q]);
+        });
+      }
     }
   }
 
