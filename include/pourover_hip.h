@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define POL_ABI_VERSION 2
+#define POL_ABI_VERSION 3
 #define POL_Q 19
 #define POL_NRED 12 /* length of the reduction vector, see POL_RED_* */
 
@@ -151,7 +151,8 @@ typedef struct pol_buffers {
    *   [0] dense index of the chunk's first cell, [1 + a + 3 b] row_base of
    *   row (y + a - 1, z + b - 1) (a, b in 0..2; a row past a non-periodic
    *   face is the chunk's own row, periodic faces wrap), [10] own row's run
-   *   x_begin | x_end << 16, [11] 0.
+   *   x_begin | x_end << 16, [11] the chunk's row y | z << 16 (so nx, ny,
+   *   nz < 65536).
    * Every stored slot is written each step (solid cells of a run with zero
    * deviations).  active_chunks must be NULL with records. */
   const uint32_t *chunk_records;

@@ -16,7 +16,7 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libpourover_hip.so"
 
 # ---- constants (mirror include/pourover_hip.h) ------------------------------
-POL_ABI_VERSION = 2
+POL_ABI_VERSION = 3
 POL_Q = 19
 POL_NRED = 12
 POL_NOALLOC_MAX_BLOCKS = 65536

@@ -115,22 +115,24 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   const StepArgs& a = A.fw;
   const uint32_t lblk = logical_block(a.per_xcd);
   if (lblk >= a.nblk) return;
-  uint32_t cell, yz, x;
+  uint32_t cell, x, y, z;
   int rb[9];  // compact layout: slot bases of the nine neighbour rows
   int sx = 0;  // compact layout: the lane's unwrapped x (its slot is rb[4] + sx)
   bool in_run = false;
   bool inside;
-  if constexpr (SPARSE)
-    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz, x,
+  if constexpr (SPARSE) {
+    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.nx, a.ny, cell, y, z, x,
                          rb, sx, in_run);
-  else
+  } else {
+    uint32_t yz;
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
+    z = fdiv(yz, a.dny);
+    y = yz - z * a.ny;
+  }
   const uint32_t N = a.N;  // dense cells
   const uint32_t P = a.P;  // population plane stride
   uint32_t own = cell;     // population slot of this cell
   if constexpr (SPARSE) own = slot_clamp(rb[4] + sx, P);
-  const uint32_t z = fdiv(yz, a.dny);
-  const uint32_t y = yz - z * a.ny;
   const uint32_t fl = inside ? a.flags[cell] : (POL_KIND_SOLID << POL_KIND_SHIFT);
   const uint32_t rf = inside ? rflags[cell] : 0u;  // loaded beside fl: one round trip for both
   const uint32_t kind = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;

@@ -7,12 +7,12 @@
 //
 // Addressing.  A link's source (forward: x - e_q; reverse: x + e_p) lies in
 // one of the nine rows (y +- 1, z +- 1) of the cell's chunk record, at column
-// x -+ 1.  The row offset rb + x is clamped once per row into [1, P - 2]: the
+// x -+ 1.  The row offset rb + x is bounded once per row (CompactRows): the
 // layout's guard slots (slots 0, 1, P - 2 and P - 1 hold no cell) keep every
-// legitimate source inside that range, so the clamp only bounds the reads of
-// links whose value is replaced.  The column shift -+ 1 is a constant address
-// offset, so a link is one load on a scalar plane base with one 32-bit lane
-// offset.  A link that reads the cell itself instead (forward: a bounced link
+// legitimate source inside [2, P - 3] as seen from rb + x, so the bound only
+// moves the reads of links whose value is replaced, and keeps them inside the
+// plane.  The column shift -+ 1 is a constant address offset, so a link is
+// one load on a scalar plane base with one 32-bit lane offset.  A link that reads the cell itself instead (forward: a bounced link
 // q reads D*_{opp q}(x); reverse: the reader of f_p(x) is x's own bounce link
 // opp p) switches the lane offset branch-free, with a flag bit as lane mask,
 // to the own slot of the other plane: still one load, one memory round trip.
@@ -29,20 +29,25 @@ namespace pol {
 template <bool A32>
 struct CompactRows {
   using AI = typename Addr<A32>::idx;  // byte offsets: 32-bit below 4 GiB of populations
-  AI ro[9];      // byte offset of row k at this x (clamped slot rb[k] + x)
-  AI own_b;      // byte offset of the own slot, at least one element
-  int lo1, hi1;  // clamp range [1, P - 2]
+  // ro[k]: byte offset of (row k's slot at this x) - 1, the slot rb[k] + x
+  // clamped by one unsigned minimum: rb[k] + x - 1 >= 0 keeps its value
+  // (every slot a kept value comes from is >= 2), anything else -- a
+  // negative base of a row that has no run there, or past the plane -- lands
+  // on P - 3.  Loads add the element back as part of their constant offset
+  // (1 - e_x forward, 1 + e_x reverse), so a link stays one VGPR offset.
+  AI ro[9];
+  AI own_b;      // byte offset of the own slot, at least two elements
+  int hi1;       // upper end of the x-face clamp range [2, P - 2]
 
   template <typename T>
   __device__ CompactRows(const int (&rb)[9], uint32_t x, uint32_t own, uint32_t PS, T*) {
     constexpr AI B = sizeof(T);
-    lo1 = 1;
     hi1 = (int)PS - 2;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) ro[k] = (AI)clampi(rb[k] + (int)x) * B;
-    own_b = (AI)(own < 1u ? 1u : own) * B;
+    for (int k = 0; k < 9; ++k) ro[k] = (AI)min((uint32_t)(rb[k] + (int)x - 1), PS - 3u) * B;
+    own_b = (AI)(own < 2u ? 2u : own) * B;
   }
-  __device__ int clampi(int s) const { return s < lo1 ? lo1 : (s > hi1 ? hi1 : s); }
+  __device__ int clampi(int s) const { return s < 2 ? 2 : (s > hi1 ? hi1 : s); }
 };
 
 // Forward (pull) sources of a cell: link q reads slot rb[(1 - e_y) + 3 (1 - e_z)]
@@ -74,19 +79,19 @@ __device__ __forceinline__ void compact_sources(const S* __restrict__ fin, uint3
       constexpr int k = (1 - EY(q)) + 3 * (1 - EZ(q));
       AI off = R.ro[k];
       if constexpr (XF && EX(q) != 0) {
-        if (xface) {  // wrapped / clamped source, folded back onto the constant - e_x
+        if (xface) {  // wrapped / clamped source, folded back onto the constant 1 - e_x
           const int dx = EX(q) > 0 ? oxm : oxp;
-          off = (AI)(R.clampi(rb[k] + (int)x + dx) + EX(q)) * B;
+          off = (AI)(R.clampi(rb[k] + (int)x + dx) + EX(q) - 1) * B;
         }
       }
       if constexpr (q != lo) off += (AI)(q - lo) * PSB;
       if constexpr (q != 0 && decltype(bounce_c)::value) {
-        const AI bounce = R.own_b + (AI)(o - lo) * PSB + (AI)((SI)EX(q) * (SI)B);
+        const AI bounce = R.own_b + (AI)(o - lo) * PSB + (AI)((SI)(EX(q) - 1) * (SI)B);
         const AI m = (AI)(SI)__builtin_amdgcn_sbfe((int)fl, q, 1);  // 0 or all ones
         off = (bounce & m) | (off & ~m);
       }
       const S* base = fin + (AI)lo * PS;
-      const S v = *((const S*)((const char*)base + off) - EX(q));
+      const S v = *((const S*)((const char*)base + off) + (1 - EX(q)));
       if constexpr (RAW) use(qc, v);
       else use(qc, C::decode(v));
     });
@@ -135,15 +140,15 @@ __device__ __forceinline__ void compact_readers(const T* __restrict__ mu, uint32
     if constexpr (XF && EX(p) != 0) {
       if (xface) {
         const int dx = EX(p) > 0 ? oxp : oxm;
-        off = (AI)(R.clampi(rb[k] + (int)x + dx) - EX(p)) * B;
+        off = (AI)(R.clampi(rb[k] + (int)x + dx) - EX(p) - 1) * B;
       }
     }
     if constexpr (p != lo) off += (AI)(p - lo) * PSB;
-    const AI bounce = R.own_b + (AI)(o - lo) * PSB - (AI)((SI)EX(p) * (SI)B);
+    const AI bounce = R.own_b + (AI)(o - lo) * PSB - (AI)((SI)(EX(p) + 1) * (SI)B);
     const AI m = (AI)(SI)__builtin_amdgcn_sbfe((int)rf, p, 1);  // normal reader: 0 -> own slot
     off = (off & m) | (bounce & ~m);
     const T* base = mu + (AI)lo * PS;
-    use(pc, *((const T*)((const char*)base + off) + EX(p)));
+    use(pc, *((const T*)((const char*)base + off) + (1 + EX(p))));
   });
 }
 

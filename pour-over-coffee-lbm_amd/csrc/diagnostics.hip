@@ -427,12 +427,12 @@ __global__ void __launch_bounds__(256) k_stats_final(const double* __restrict__ 
 // list; the compact layout's slot comes from the chunk record.
 template <bool SPARSE>
 __device__ inline bool work_cell(const StepArgs& a, uint32_t lblk, uint32_t& cell, uint32_t& own, uint32_t& fl) {
-  uint32_t yz, x;
+  uint32_t yz, x, y, z;
   int rb[9];
   int sx = 0;
   bool in_run = false, inside;
   if constexpr (SPARSE)
-    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz, x, rb,
+    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.nx, a.ny, cell, y, z, x, rb,
                          sx, in_run);
   else
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);

@@ -264,14 +264,15 @@ __device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t
 
 // Work item -> cell for the compact layout: lane (item & (2^cshift - 1)) of
 // the chunk described by records[item >> cshift] (POL_RECORD_WORDS words:
-// dense first cell, the nine neighbour-row bases, the own row's stored run).
+// dense first cell, the nine neighbour-row bases, the own row's stored run,
+// the row's y | z << 16).
 // rb[i] is the slot base of row (y + i%3 - 1, z + i/3 - 1).  sx = x0 + lane
 // is the lane's x counted from its chunk's row without wrapping (a stored run
 // may extend past the row end as padding), and in_run tells whether sx lies
 // in the own row's stored run, i.e. whether the lane owns slot rb[4] + sx.
 __device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ records,
-                                   uint32_t cshift, uint32_t N, const FastDiv& dnx, uint32_t nx, uint32_t& cell,
-                                   uint32_t& yz, uint32_t& x, int (&rb)[9], int& sx, bool& in_run) {
+                                   uint32_t cshift, uint32_t N, uint32_t nx, uint32_t ny, uint32_t& cell,
+                                   uint32_t& y, uint32_t& z, uint32_t& x, int (&rb)[9], int& sx, bool& in_run) {
   const bool listed = item < n_items;
   bool ok = listed;
   const uint32_t lane = item & ((1u << cshift) - 1u);
@@ -283,22 +284,20 @@ __device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32
     w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
     w[8] = c.x; w[9] = c.y; w[10] = c.z; w[11] = c.w;
   }
-  const uint32_t yz0 = fdiv(w[0], dnx);
-  sx = (int)(w[0] - yz0 * nx + lane);
-  uint32_t cc = w[0] + lane;
-  ok = ok && cc < N;
-  if (!ok) cc = 0;
-  yz = fdiv(cc, dnx);
-  x = cc - yz * nx;
-  if (x < lane) ok = false;  // wrapped into the next row
+  // the chunk's row (y, z) from word 11: no division per lane
+  const uint32_t y0 = w[11] & 0xffffu, z0 = w[11] >> 16;
+  const uint32_t x0 = w[0] - (z0 * ny + y0) * nx;
+  sx = (int)(x0 + lane);
+  x = x0 + lane;
+  const uint32_t cc = w[0] + lane;
+  ok = ok && x < nx && cc < N;  // lanes past the row end are padding
 #pragma unroll
   for (int i = 0; i < 9; ++i) rb[i] = (int)w[1 + i];
   in_run = listed && sx >= (int)(w[10] & 0xffffu) && sx < (int)(w[10] >> 16);
   cell = ok ? cc : 0u;
-  if (!ok) {
-    yz = 0;
-    x = 0;
-  }
+  y = ok ? y0 : 0u;
+  z = ok ? z0 : 0u;
+  if (!ok) x = 0;
   return ok;
 }
 

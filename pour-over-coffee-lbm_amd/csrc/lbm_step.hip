@@ -74,12 +74,12 @@ __global__ void __launch_bounds__(BLOCK) k_init_equilibrium_sparse(const StepArg
   using R = typename C::real;
   const uint32_t lblk = ordered_block(a.order, a.per_xcd);
   if (lblk >= a.nblk) return;
-  uint32_t cell, yz, x;
+  uint32_t cell, x, y, z;
   int rb[9];
   int sx = 0;
   bool in_run = false;
-  const bool ok = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz,
-                              x, rb, sx, in_run);
+  const bool ok = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.nx, a.ny, cell, y,
+                              z, x, rb, sx, in_run);
   if (!in_run) return;
   const uint32_t N = a.N, P = a.P;
   const uint32_t own = slot_clamp(rb[4] + sx, P);

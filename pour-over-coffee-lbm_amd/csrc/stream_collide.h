@@ -61,18 +61,20 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   using AI = typename AD::idx;
   const uint32_t lblk = ordered_block(a.order, a.per_xcd);
   if (lblk >= a.nblk) return;  // whole padding block exits (uniform)
-  uint32_t cell, yz, x;
+  uint32_t cell, x, y, z;
   int rb[9];          // compact layout: slot bases of the nine neighbour rows
   int sx = 0;         // compact layout: the lane's unwrapped x (its slot is rb[4] + sx)
   bool in_run = false;
   bool inside;
-  if constexpr (SPARSE)
-    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.dnx, a.nx, cell, yz, x,
+  if constexpr (SPARSE) {
+    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.nx, a.ny, cell, y, z, x,
                          rb, sx, in_run);
-  else
+  } else {
+    uint32_t yz;
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
-  const uint32_t z = fdiv(yz, a.dny);
-  const uint32_t y = yz - z * a.ny;
+    z = fdiv(yz, a.dny);
+    y = yz - z * a.ny;
+  }
 
   const uint32_t fl = FLAGS ? (inside ? a.flags[cell] : (POL_KIND_SOLID << POL_KIND_SHIFT)) : 0u;
   const uint32_t kind = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;

@@ -27,7 +27,8 @@ RECORD_WORDS uint32 per 32-cell row chunk that meets a run,
             the chunk's own row (the clamp the gather applies), periodic
             faces wrap
     [10]    own row's run: x_begin | x_end << 16
-    [11]    0
+    [11]    the chunk's row: y | z << 16 (the kernels take (x, y, z) from it
+            without a division)
 so a thread finds every neighbour slot from its record without further
 lookups: source of link q = word[1 + (1 - e_y) + 3 (1 - e_z)] + x - e_x
 (x wrapped or clamped like the dense gather).  With align > 1 each row's
@@ -110,8 +111,8 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
     nz, ny, nx = flags.shape
     if chunk_cells % align:
         raise ValueError("align must divide chunk_cells")
-    if nx + align >= 1 << 16:
-        raise ValueError("the compact layout packs x in 16 bits (nx < 65536)")
+    if nx + align >= 1 << 16 or ny >= 1 << 16 or nz >= 1 << 16:
+        raise ValueError("the compact layout packs x, y and z in 16 bits (below 65536)")
     kind = (flags >> np.uint32(L.POL_KIND_SHIFT)) & np.uint32(L.POL_KIND_MASK)
     open_cell = (kind != L.POL_KIND_SOLID).reshape(nz * ny, nx)
     has = open_cell.any(axis=1)
@@ -163,6 +164,7 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
             y2 = np.where((y2 < 0) | (y2 >= ny), (y2 % ny) if py else yy, y2)
             rec[:, 1 + a + 3 * b] = row_base[z2 * ny + y2]
     rec[:, 10] = row_run[rr]
+    rec[:, 11] = yy | (zz << 16)
     records = (rec & 0xFFFFFFFF).astype(np.uint32)  # row bases as two's-complement int32
     x_face = bool((has & ((xb == 0) | (xe == nx))).any())
     return CompactLayout(nx, ny, nz, M, used, row_base, row_run, records, slot_dense, align, x_face)

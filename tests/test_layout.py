@@ -18,18 +18,22 @@ def _lanes(lay, chunk_cells=32, with_padding=False):
     with their unwrapped x (record row, sx, y, z, valid)."""
     rec = lay.records
     base = rec[:, 0].astype(np.int64)
+    # the kernels' decode (lbm_common.h item_record): the row from word 11,
+    # x0 = first - row nx, lanes at x0 + lane, valid below nx
+    y0 = (rec[:, 11] & 0xFFFF).astype(np.int64)
+    z0 = (rec[:, 11] >> 16).astype(np.int64)
+    assert np.array_equal(z0 * lay.ny + y0, base // lay.nx)  # word 11 is the first cell's row
+    x0 = base - (z0 * lay.ny + y0) * lay.nx
     lane = np.arange(chunk_cells, dtype=np.int64)
     cell = base[:, None] + lane[None, :]
-    yz = cell // lay.nx
-    x = cell % lay.nx
-    ok = (x >= lane[None, :]) & (cell < lay.nx * lay.ny * lay.nz)
+    sx = x0[:, None] + lane[None, :]
+    ok = (sx < lay.nx) & (cell < lay.nx * lay.ny * lay.nz)
     r = np.broadcast_to(np.arange(rec.shape[0])[:, None], cell.shape)
+    yb = np.broadcast_to(y0[:, None], cell.shape)
+    zb = np.broadcast_to(z0[:, None], cell.shape)
     if with_padding:
-        yz0 = base // lay.nx
-        sx = (base - yz0 * lay.nx)[:, None] + lane[None, :]
-        return (r.ravel(), sx.ravel(), np.broadcast_to((yz0 % lay.ny)[:, None], cell.shape).ravel(),
-                np.broadcast_to((yz0 // lay.ny)[:, None], cell.shape).ravel(), ok.ravel())
-    return r[ok], x[ok], (yz % lay.ny)[ok], (yz // lay.ny)[ok]
+        return r.ravel(), sx.ravel(), yb.ravel(), zb.ravel(), ok.ravel()
+    return r[ok], sx[ok], yb[ok], zb[ok]
 
 
 LAYOUTS = [(1, 32), (32, 32), (16, 16), (32, 64), (64, 64)]
@@ -97,13 +101,16 @@ def test_gather_slots_match_dense_sources(name, align, chunk):
         m = sel & ~bounce
         assert (kind[zs[m], ys[m], xs[m]] != L.POL_KIND_SOLID).all(), q
         assert np.array_equal(slot[m], lay.slot_of(xs[m], ys[m], zs[m])), q
-        # the kernel's fast form for lanes off the x faces: the row offset
-        # clamped once into [1, M - 2], the x shift a constant; every link
-        # (bounce links too, whose value is replaced) stays inside the plane
+        # the kernel's fast form for lanes off the x faces (CompactRows): the
+        # row offset bounded once by an unsigned minimum, ro = min(u32(rb + x
+        # - 1), M - 3), the load at ro + 1 - e_x; every link (bounce links
+        # too, whose value is replaced) stays inside the plane
         fast = sel & (x > 0) & (x < nx - 1)
-        ro = np.clip(_rb(lay, r, 1 - ey, 1 - ez) + x, 1, lay.stored_cells - 2)
-        assert np.array_equal((ro - ex)[fast & ~bounce], slot[fast & ~bounce]), q
-        assert (ro - ex).min() >= 0 and (ro - ex).max() <= lay.stored_cells - 1
+        M = lay.stored_cells
+        ro = np.minimum((_rb(lay, r, 1 - ey, 1 - ez) + x - 1) & 0xFFFFFFFF, M - 3)
+        idx = ro + 1 - ex
+        assert np.array_equal(idx[fast & ~bounce], slot[fast & ~bounce]), q
+        assert idx.min() >= 0 and idx.max() <= M - 1
 
 
 @pytest.mark.parametrize("align,chunk", LAYOUTS)
