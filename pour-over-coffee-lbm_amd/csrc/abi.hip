@@ -141,7 +141,10 @@ int pol_create(pol_ctx** out, const pol_desc* desc) {
   // ~8 groups of four cells: the per-block reduction of the 43 statistics is
   // amortised; 224^3 V60 kernel 0.062 ms against 0.069 / 0.067 ms with 8 / 24)
   const uint64_t nxy = (uint64_t)d.nx * d.ny;
-  c->stat_tiles = (uint32_t)std::min<uint64_t>((nxy + BLOCK - 1) / BLOCK, 12);
+  // 12 strips per plane; more for planes so large that a lane would take
+  // 65536 cells (the statistics kernel counts in 16-bit halves)
+  c->stat_tiles = (uint32_t)std::max<uint64_t>(std::min<uint64_t>((nxy + BLOCK - 1) / BLOCK, 12),
+                                               (nxy + 128ull * 65000 - 1) / (128ull * 65000));
   const size_t sblk = (size_t)c->stat_tiles * d.nz;
   if (e == hipSuccess) e = hipMalloc((void**)&c->stat_partial, sblk * POL_NSTAT * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&c->stat_zpart, sblk * sizeof(double));

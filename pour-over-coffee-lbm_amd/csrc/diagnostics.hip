@@ -154,16 +154,44 @@ __device__ inline float mach_of(float x) {
   return __builtin_isinf(x) ? x : q;
 }
 
-// Role A: rho and the velocity components.
+// Extremes of |u| as integer keys.  |u| = sqrt((ux*ux + uy*uy) + uz*uz) is
+// +0, positive, +inf or NaN (never -0: the squares are >= +0), so the key
+// (bits << 1) + 1 (sign bit dropped) orders the values as the floats do,
+// puts every NaN above +inf, and leaves 0 free for "no value"; one integer
+// max or min per statistic replaces an FP64 max with its NaN handling.
+constexpr uint32_t UKEY_INF = (0x7f800000u << 1) + 1u;
+__device__ inline uint32_t umag_key(float um) { return (__float_as_uint(um) << 1) + 1u; }
+__device__ inline double umag_of_key(uint32_t k) { return (double)__uint_as_float(k >> 1); }
+
+// Wave reduction of a key statistic (max or min of keys) into sh[wid][S],
+// decoded: an empty key gives the statistic's identity, a NaN key NaN.
+template <int S, bool MIN>
+__device__ inline void wave_key(uint32_t k, double (*sh)[POL_NSTAT + 1], int wid, int lane) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = __shfl_xor(k, o, 64);
+    k = MIN ? min(k, t) : max(k, t);
+  }
+  if (lane == 0) {
+    const bool empty = MIN ? k == ~0u : k == 0u;
+    sh[wid][S] = empty ? stat_identity(S) : (k > UKEY_INF ? (double)NAN : umag_of_key(k));
+  }
+}
+
+// Role A: rho and the velocity components.  Branch-free per cell: the
+// non-solid statistics take selected values (the identity where a cell does
+// not count), so mixed solid / fluid waves run one path.
+template <typename R>
 struct StatsA {
   double s_d = 0.0, s_d2 = 0.0, mx_d = -1.0e300, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, s_ux = 0.0, s_uy = 0.0,
-         s_uz = 0.0, s_ux2 = 0.0, s_uy2 = 0.0, s_uz2 = 0.0, down = 0.0, zs = 0.0, r_min = 1.0e300, r_max = -1.0e300;
-  uint32_t cells = 0, fluid = 0, r_nan = 0, r_inf = 0;
-  template <typename R>
+         s_uz = 0.0, s_ux2 = 0.0, s_uy2 = 0.0, s_uz2 = 0.0, down = 0.0;
+  R r_min = R(INFINITY), r_max = R(-INFINITY);  // over finite rho of non-solid cells (storage type: exact)
+  // counts in 16-bit halves (a lane takes fewer than 65536 cells): cells |
+  // non-solid << 16, NaN rho | infinite rho << 16
+  uint32_t c_cells = 0, c_nonfinite = 0;
   __device__ void add(R r, R ux, R uy, R uz, uint32_t fl, bool outlet) {
     const double rd = (double)r;
     const double dux = (double)ux, duy = (double)uy, duz = (double)uz;
-    cells += 1u;
     const double d = rd - 1.0;
     s_d += d;
     s_d2 += d * d;
@@ -177,20 +205,18 @@ struct StatsA {
     s_ux2 += dux * dux;
     s_uy2 += duy * duy;
     s_uz2 += duz * duz;
-    if (outlet) down += fmax(-duz, 0.0);
-    zs += rd;
-    if (((fl >> POL_KIND_SHIFT) & POL_KIND_MASK) != POL_KIND_SOLID) {
-      fluid += 1u;
-      if (isnan(rd)) r_nan += 1u;
-      else if (isinf(rd)) r_inf += 1u;
-      else {
-        r_min = fmin(r_min, rd);
-        r_max = fmax(r_max, rd);
-      }
-    }
+    if (outlet) down += fmax(-duz, 0.0);  // block-uniform
+    const bool ns = ((fl >> POL_KIND_SHIFT) & POL_KIND_MASK) != POL_KIND_SOLID;
+    const bool nan = __builtin_isnan(r), inf = __builtin_isinf(r);
+    c_cells += ns ? 0x10001u : 1u;
+    c_nonfinite += (ns && nan) ? 1u : 0u;
+    c_nonfinite += (ns && inf) ? 0x10000u : 0u;
+    const bool fin = ns && __builtin_isfinite(r);
+    r_min = fmin(r_min, fin ? r : R(INFINITY));
+    r_max = fmax(r_max, fin ? r : R(-INFINITY));
   }
   __device__ void reduce(double (*sh)[POL_NSTAT + 1], int wid, int lane) {
-    wave_stat<POL_STAT_CELLS>(0.0, cells, sh, wid, lane);
+    wave_stat<POL_STAT_CELLS>(0.0, c_cells & 0xffffu, sh, wid, lane);
     wave_stat<POL_STAT_SUM_DRHO>(s_d, 0u, sh, wid, lane);
     wave_stat<POL_STAT_SUM_DRHO2>(s_d2, 0u, sh, wid, lane);
     wave_stat<POL_STAT_MAX_ABS_DRHO>(mx_d, 0u, sh, wid, lane);
@@ -204,88 +230,100 @@ struct StatsA {
     wave_stat<POL_STAT_SUM_UY2>(s_uy2, 0u, sh, wid, lane);
     wave_stat<POL_STAT_SUM_UZ2>(s_uz2, 0u, sh, wid, lane);
     wave_stat<POL_STAT_OUTLET_DOWN>(down, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID>(0.0, fluid, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID_RHO_NAN>(0.0, r_nan, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID_RHO_INF>(0.0, r_inf, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID_RHO_MIN>(r_min, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID_RHO_MAX>(r_max, 0u, sh, wid, lane);
-    const double z = wave_sum(zs);
-    if (lane == 0) sh[wid][POL_NSTAT] = z;
+    wave_stat<POL_STAT_FLUID>(0.0, c_cells >> 16, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_RHO_NAN>(0.0, c_nonfinite & 0xffffu, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_RHO_INF>(0.0, c_nonfinite >> 16, sh, wid, lane);
+    // +-inf only as the identity (the values are finite)
+    const double lo = r_min == R(INFINITY) ? 1.0e300 : (double)r_min;
+    const double hi = r_max == R(-INFINITY) ? -1.0e300 : (double)r_max;
+    wave_stat<POL_STAT_FLUID_RHO_MIN>(lo, 0u, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_RHO_MAX>(hi, 0u, sh, wid, lane);
+    // the plane's density sum: sum (rho - 1) plus the cell count (exact),
+    // taken from the two statistics just reduced
+    if (lane == 0) sh[wid][POL_NSTAT] = sh[wid][POL_STAT_SUM_DRHO] + sh[wid][POL_STAT_CELLS];
   }
 };
 
-// Role B: |u| and what derives from it, and the temporal change.
+// Role B: |u| and what derives from it, and the temporal change.  Branch-free
+// per cell like role A; the extremes of |u| are integer keys (umag_key).
+// EXTRA: a previous state or a phase field is given (without one, the
+// temporal change and the air mass are zero and every cell is water, and
+// their accumulators are not kept).
+template <bool EXTRA>
 struct StatsB {
-  double s_um = 0.0, s_um2 = 0.0, mx_um = -1.0e300, s_mach = 0.0, s_uma = 0.0, p_r2 = 0.0, p_u2 = 0.0, f_umax = 0.0,
-         i_water = 0.0, i_air = 0.0, i_umax = -1.0e300, i_umin = 1.0e300, i_usum = 0.0;
-  uint32_t active = 0, mach_gt = 0, um_active = 0, u_nan = 0, u_inf = 0, i_cells = 0;
-  // i: the cell's index within plane z (base = z nxy); zin: 1 <= z <= nz - 2
+  double s_um = 0.0, s_um2 = 0.0, s_mach = 0.0, s_uma = 0.0, p_r2 = 0.0, p_u2 = 0.0, i_water = 0.0, i_air = 0.0,
+         i_usum = 0.0;
+  uint32_t k_max = 0u, k_fmax = 0u, k_imax = 0u, k_imin = ~0u;  // MAX_UMAG, FLUID_UMAX, INT_UMAX, INT_UMIN
+  // counts in 16-bit halves (a lane takes fewer than 65536 cells):
+  // Mach > 1e-10 | Mach > 0.1 << 16, |u| > 1e-10 | interior << 16,
+  // NaN |u| | infinite |u| << 16 (non-solid cells)
+  uint32_t c_mach = 0, c_um = 0, c_nonfinite = 0;
+  // i: the cell's index within plane z (base = z nxy); in_box: 1 <= x, y, z
+  // <= n - 2 (the interior test without the solid check)
   template <typename R>
-  __device__ void add(const StatsArgs& a, size_t base, uint32_t i, bool zin, R r, R ux, R uy, R uz, uint32_t fl) {
+  __device__ void add(const StatsArgs& a, size_t base, uint32_t i, bool in_box, R r, R ux, R uy, R uz, uint32_t fl) {
     const double rd = (double)r;
     const float umf = umag_f32(ux, uy, uz);
     const double um = (double)umf;
+    const uint32_t key = umag_key(umf);
     s_um += um;
     s_um2 += um * um;
-    mx_um = nanmax(mx_um, um);
+    k_max = max(k_max, key);
+    // sums over Mach > 1e-10 and |u| > 1e-10 add +0 for the other cells
+    // (a sum of non-negative values that starts at +0 is unchanged by it)
     const float mach = mach_of(umf);
-    if (mach > 1e-10f) {
-      s_mach += (double)mach;
-      active += 1u;
-    }
-    if (mach > 0.1f) mach_gt += 1u;
-    if (umf > 1e-10f) {
-      s_uma += um;
-      um_active += 1u;
-    }
-    if (a.prev_rho) {
-      const double e = rd - (double)((const R*)a.prev_rho)[base + i];
-      p_r2 += e * e;
-    }
-    if (a.prev_u) {
-      const R* pu = (const R*)a.prev_u + base + i;
-      const double ex = (double)ux - (double)pu[0], ey = (double)uy - (double)pu[a.N],
-                   ez = (double)uz - (double)pu[2 * (size_t)a.N];
-      p_u2 += ex * ex + ey * ey + ez * ez;
-    }
-    if (((fl >> POL_KIND_SHIFT) & POL_KIND_MASK) != POL_KIND_SOLID) {
-      if (isnan(um)) u_nan += 1u;
-      else if (isinf(um)) u_inf += 1u;
-      else f_umax = fmax(f_umax, um);
-      const uint32_t y = fdiv(i, a.dnx), x = i - y * a.nx;
-      if (zin && x >= 1 && x + 1 < a.nx && y >= 1 && y + 1 < a.ny) {
-        i_cells += 1u;
-        const bool water = a.phase ? (((const R*)a.phase)[base + i] > R(0.1)) : true;
-        // selects, not a branch on which accumulator (adding +0 leaves a sum
-        // that starts at +0 unchanged)
-        i_water += water ? rd : 0.0;
-        i_air += water ? 0.0 : rd;
-        i_umax = fmax(i_umax, um);
-        i_umin = fmin(i_umin, um);
-        i_usum += um;
+    const bool act = mach > 1e-10f, ua = umf > 1e-10f;
+    s_mach += act ? (double)mach : 0.0;
+    c_mach += (act ? 1u : 0u) + (mach > 0.1f ? 0x10000u : 0u);
+    s_uma += ua ? um : 0.0;
+    bool water = true;
+    if constexpr (EXTRA) {
+      if (a.prev_rho) {
+        const double e = rd - (double)((const R*)a.prev_rho)[base + i];
+        p_r2 += e * e;
       }
+      if (a.prev_u) {
+        const R* pu = (const R*)a.prev_u + base + i;
+        const double ex = (double)ux - (double)pu[0], ey = (double)uy - (double)pu[a.N],
+                     ez = (double)uz - (double)pu[2 * (size_t)a.N];
+        p_u2 += ex * ex + ey * ey + ez * ez;
+      }
+      if (a.phase) water = ((const R*)a.phase)[base + i] > R(0.1);
     }
+    const bool ns = ((fl >> POL_KIND_SHIFT) & POL_KIND_MASK) != POL_KIND_SOLID;
+    c_nonfinite += (ns && key > UKEY_INF) ? 1u : 0u;
+    c_nonfinite += (ns && key == UKEY_INF) ? 0x10000u : 0u;
+    k_fmax = max(k_fmax, (ns && key < UKEY_INF) ? key : 0u);
+    const bool in = ns && in_box;
+    c_um += (ua ? 1u : 0u) + (in ? 0x10000u : 0u);
+    i_water += (in && water) ? rd : 0.0;
+    if constexpr (EXTRA) i_air += (in && !water) ? rd : 0.0;
+    i_usum += in ? um : 0.0;
+    // interior max / min ignore NaN |u| (they keep +inf)
+    const bool iv = in && key <= UKEY_INF;
+    k_imax = max(k_imax, iv ? key : 0u);
+    k_imin = min(k_imin, iv ? key : ~0u);
   }
   __device__ void reduce(double (*sh)[POL_NSTAT + 1], int wid, int lane) {
     wave_stat<POL_STAT_SUM_UMAG>(s_um, 0u, sh, wid, lane);
     wave_stat<POL_STAT_SUM_UMAG2>(s_um2, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_MAX_UMAG>(mx_um, 0u, sh, wid, lane);
+    wave_key<POL_STAT_MAX_UMAG, false>(k_max, sh, wid, lane);
     wave_stat<POL_STAT_SUM_MACH_ACTIVE>(s_mach, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_ACTIVE>(0.0, active, sh, wid, lane);
-    wave_stat<POL_STAT_MACH_GT>(0.0, mach_gt, sh, wid, lane);
+    wave_stat<POL_STAT_ACTIVE>(0.0, c_mach & 0xffffu, sh, wid, lane);
+    wave_stat<POL_STAT_MACH_GT>(0.0, c_mach >> 16, sh, wid, lane);
     wave_stat<POL_STAT_DRHO_PREV2>(p_r2, 0u, sh, wid, lane);
     wave_stat<POL_STAT_DU_PREV2>(p_u2, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID_U_NAN>(0.0, u_nan, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID_U_INF>(0.0, u_inf, sh, wid, lane);
-    wave_stat<POL_STAT_FLUID_UMAX>(f_umax, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_INT_CELLS>(0.0, i_cells, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_U_NAN>(0.0, c_nonfinite & 0xffffu, sh, wid, lane);
+    wave_stat<POL_STAT_FLUID_U_INF>(0.0, c_nonfinite >> 16, sh, wid, lane);
+    wave_key<POL_STAT_FLUID_UMAX, false>(k_fmax, sh, wid, lane);
+    wave_stat<POL_STAT_INT_CELLS>(0.0, c_um >> 16, sh, wid, lane);
     wave_stat<POL_STAT_INT_WATER>(i_water, 0u, sh, wid, lane);
     wave_stat<POL_STAT_INT_AIR>(i_air, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_INT_UMAX>(i_umax, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_INT_UMIN>(i_umin, 0u, sh, wid, lane);
+    wave_key<POL_STAT_INT_UMAX, false>(k_imax, sh, wid, lane);
+    wave_key<POL_STAT_INT_UMIN, true>(k_imin, sh, wid, lane);
     wave_stat<POL_STAT_INT_USUM>(i_usum, 0u, sh, wid, lane);
     wave_stat<POL_STAT_SUM_UMAG_ACTIVE>(s_uma, 0u, sh, wid, lane);
-    wave_stat<POL_STAT_UMAG_ACTIVE>(0.0, um_active, sh, wid, lane);
+    wave_stat<POL_STAT_UMAG_ACTIVE>(0.0, c_um & 0xffffu, sh, wid, lane);
   }
 };
 
@@ -310,7 +348,7 @@ struct Group4 {
 template <typename R, bool VEC>
 __device__ inline void stats_role_a(const StatsArgs& a, uint32_t z, uint32_t t0, uint32_t stride,
                                     double (*sh)[POL_NSTAT + 1], int wid, int lane) {
-  StatsA acc;
+  StatsA<R> acc;
   const bool outlet = (int)z < a.z_out;
   const size_t base = (size_t)z * a.nxy;
   const R* rho = (const R*)a.rho + base;
@@ -324,19 +362,19 @@ __device__ inline void stats_role_a(const StatsArgs& a, uint32_t z, uint32_t t0,
       Group4<R> g;
       g.load(rho, ux_p, uy_p, uz_p, flp, j, fluid_word);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) acc.add<R>(g.r[k], g.x[k], g.y[k], g.w[k], g.f[k], outlet);
+      for (int k = 0; k < 4; ++k) acc.add(g.r[k], g.x[k], g.y[k], g.w[k], g.f[k], outlet);
     }
   } else {
     for (uint32_t i = t0; i < a.nxy; i += stride)
-      acc.add<R>(rho[i], ux_p[i], uy_p[i], uz_p[i], flp ? flp[i] : fluid_word, outlet);
+      acc.add(rho[i], ux_p[i], uy_p[i], uz_p[i], flp ? flp[i] : fluid_word, outlet);
   }
   acc.reduce(sh, wid, lane);
 }
 
-template <typename R, bool VEC>
+template <typename R, bool VEC, bool EXTRA>
 __device__ inline void stats_role_b(const StatsArgs& a, uint32_t z, uint32_t t0, uint32_t stride,
                                     double (*sh)[POL_NSTAT + 1], int wid, int lane) {
-  StatsB acc;
+  StatsB<EXTRA> acc;
   const size_t base = (size_t)z * a.nxy;
   const R* rho = (const R*)a.rho + base;
   const R* ux_p = (const R*)a.u + base;
@@ -345,22 +383,38 @@ __device__ inline void stats_role_b(const StatsArgs& a, uint32_t z, uint32_t t0,
   const uint32_t* flp = a.flags ? a.flags + base : nullptr;
   const uint32_t fluid_word = (uint32_t)(POL_KIND_FLUID << POL_KIND_SHIFT);
   const bool zin = z >= 1 && z + 1 < a.nz;
+  // 1 <= x <= nx - 2 and 1 <= y <= ny - 2 as unsigned range tests
+  const uint32_t nx2 = a.nx - 2u, ny2 = a.ny - 2u;
   if constexpr (VEC) {
+    // the launcher takes this form only for nx >= 4: a group's four cells
+    // then span at most two rows
     for (uint32_t j = t0; j < a.nxy / 4; j += stride) {
       Group4<R> g;
       g.load(rho, ux_p, uy_p, uz_p, flp, j, fluid_word);
+      const uint32_t y0 = fdiv(4 * j, a.dnx), x0 = 4 * j - y0 * a.nx;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) acc.add<R>(a, base, 4 * j + k, zin, g.r[k], g.x[k], g.y[k], g.w[k], g.f[k]);
+      for (int k = 0; k < 4; ++k) {
+        const bool wrap = x0 + k >= a.nx;
+        const uint32_t x = wrap ? x0 + k - a.nx : x0 + k, y = wrap ? y0 + 1 : y0;
+        const bool in_box = zin && x - 1u < nx2 && y - 1u < ny2;
+        acc.template add<R>(a, base, 4 * j + k, in_box, g.r[k], g.x[k], g.y[k], g.w[k], g.f[k]);
+      }
     }
   } else {
-    for (uint32_t i = t0; i < a.nxy; i += stride)
-      acc.add<R>(a, base, i, zin, rho[i], ux_p[i], uy_p[i], uz_p[i], flp ? flp[i] : fluid_word);
+    for (uint32_t i = t0; i < a.nxy; i += stride) {
+      const uint32_t y = fdiv(i, a.dnx), x = i - y * a.nx;
+      const bool in_box = zin && x - 1u < nx2 && y - 1u < ny2;
+      acc.template add<R>(a, base, i, in_box, rho[i], ux_p[i], uy_p[i], uz_p[i], flp ? flp[i] : fluid_word);
+    }
   }
   acc.reduce(sh, wid, lane);
 }
 
-template <typename R, bool VEC>
-__global__ void __launch_bounds__(BLOCK) k_field_stats(const StatsArgs a) {
+// FP32 fields without a previous state or phase: at most 64 VGPRs (eight
+// waves per SIMD to cover the load latency); the other forms hold more
+// (FP64 fields twice the loaded bytes) and keep the compiler's allocation
+template <typename R, bool VEC, bool EXTRA>
+__global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && !EXTRA) ? 8 : 1) k_field_stats(const StatsArgs a) {
   static_assert(BLOCK == 256, "two wave pairs per block");
   const uint32_t z = blockIdx.y;
   __shared__ double sh[BLOCK / 64][POL_NSTAT + 1];
@@ -368,7 +422,7 @@ __global__ void __launch_bounds__(BLOCK) k_field_stats(const StatsArgs a) {
   const uint32_t t0 = blockIdx.x * (BLOCK / 2) + (threadIdx.x & (BLOCK / 2 - 1));
   const uint32_t stride = a.tiles * (BLOCK / 2);
   if (wid < 2) stats_role_a<R, VEC>(a, z, t0, stride, sh, wid, lane);
-  else stats_role_b<R, VEC>(a, z, t0, stride, sh, wid, lane);
+  else stats_role_b<R, VEC, EXTRA>(a, z, t0, stride, sh, wid, lane);
   __syncthreads();
   const size_t blk = (size_t)z * a.tiles + blockIdx.x;
   if (threadIdx.x < POL_NSTAT) {
@@ -544,20 +598,26 @@ __global__ void __launch_bounds__(256) k_export_fields(const R* __restrict__ rho
 }
 
 // ---- launchers ------------------------------------------------------------
+template <typename R, bool VEC>
+static void launch_stats(bool extra, dim3 grid, const StatsArgs& s, hipStream_t st) {
+  if (extra) hipLaunchKernelGGL((k_field_stats<R, VEC, true>), grid, dim3(BLOCK), 0, st, s);
+  else hipLaunchKernelGGL((k_field_stats<R, VEC, false>), grid, dim3(BLOCK), 0, st, s);
+}
+
 hipError_t launch_field_stats(bool real_double, const StatsArgs& s, double* out_dev, double* zout_dev,
                               hipStream_t st) {
   const dim3 grid(s.tiles, s.nz);
   // 16-byte loads need every plane (and so every u component) to start on a
-  // 4-cell boundary
-  const bool vec = s.nxy % 4 == 0;
-  if (real_double && vec)
-    hipLaunchKernelGGL((k_field_stats<double, true>), grid, dim3(BLOCK), 0, st, s);
-  else if (real_double)
-    hipLaunchKernelGGL((k_field_stats<double, false>), grid, dim3(BLOCK), 0, st, s);
-  else if (vec)
-    hipLaunchKernelGGL((k_field_stats<float, true>), grid, dim3(BLOCK), 0, st, s);
-  else
-    hipLaunchKernelGGL((k_field_stats<float, false>), grid, dim3(BLOCK), 0, st, s);
+  // 4-cell boundary; nx >= 4 keeps a 4-cell group within two rows
+  const bool vec = s.nxy % 4 == 0 && s.nx >= 4;
+  const bool extra = s.prev_rho || s.prev_u || s.phase;
+  if (real_double) {
+    if (vec) launch_stats<double, true>(extra, grid, s, st);
+    else launch_stats<double, false>(extra, grid, s, st);
+  } else {
+    if (vec) launch_stats<float, true>(extra, grid, s, st);
+    else launch_stats<float, false>(extra, grid, s, st);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_stats_final, dim3(POL_NSTAT + s.nz), dim3(256), 0, st, s.partial, s.zpart, s.tiles * s.nz,
