@@ -89,6 +89,7 @@ class AdjointProblem:
         fshape = tuple(self.ctx.f[0].shape)
         self.ckpt = None if self.store_all else torch.empty((nseg,) + fshape, dtype=sd, device=dev)
         self.seg = torch.empty((checkpoint,) + fshape, dtype=sd, device=dev)
+        self._seg_holds = None  # the segment whose states self.seg holds (the forward run leaves the last one)
         self.timings = {}
 
     # ---- parameters -------------------------------------------------------------
@@ -105,11 +106,14 @@ class AdjointProblem:
     def forward(self, checkpoints: bool = False) -> float:
         """J (FP64) of one full run from rest; optionally store the checkpoints
         (or, with checkpoint=None, every state: the run then steps from one
-        stored state into the next)."""
+        stored state into the next).  With checkpoints every K steps the run
+        steps through the segment store over its last segment, so the reverse
+        sweep starts without rebuilding it."""
         ctx = self.ctx
         ctx.set_current(0)
         ctx.init_equilibrium()
         J = 0.0
+        self._seg_holds = None
         if checkpoints and self.store_all:
             seg = self.seg
             seg[0].copy_(ctx.f_current)
@@ -120,9 +124,26 @@ class AdjointProblem:
                 if what:
                     J += float(red[7])
             return J
+        K = self.checkpoint
+        last = self.steps - K  # first step of the last segment
         for n in range(self.steps):
-            if checkpoints and n % self.checkpoint == 0:
-                self.ckpt[n // self.checkpoint].copy_(ctx.f_current)
+            if checkpoints and n % K == 0:
+                self.ckpt[n // K].copy_(ctx.f_current)
+            if checkpoints and n >= last:
+                # f^n .. f^{steps-1} into the segment store, the final state
+                # into the context's next buffer (the same fused step as
+                # ctx.step, from and into other buffers)
+                seg = self.seg
+                if n == last:
+                    seg[0].copy_(ctx.f_current)
+                out = seg[n - last + 1] if n + 1 < self.steps else ctx.f_next
+                what = L.POL_WANT_REDUCE if self.objective_step(n) else 0
+                red = ctx.step_from(seg[n - last], out, what)
+                if what:
+                    J += float(red[7])
+                if n + 1 == self.steps:
+                    self._seg_holds = self.steps // K - 1
+                continue
             if self.objective_step(n):
                 red = ctx.step(1, L.POL_WANT_REDUCE)
                 J += float(red[7])
@@ -181,8 +202,9 @@ class AdjointProblem:
         """Adjoint steps from the last step to the first (after forward with
         checkpoints and pol_adjoint_reset), enqueued on the current stream."""
         for s in reversed(range(self.steps // self.checkpoint)):
-            if not self.store_all:
+            if not self.store_all and self._seg_holds != s:
                 self._rebuild(s, self.seg)  # f^{sK} .. f^{sK+K-1}
+                self._seg_holds = s
             self._sweep_segment(s, self.seg)
 
 
