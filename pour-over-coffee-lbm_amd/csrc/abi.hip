@@ -37,7 +37,7 @@ struct pol_ctx {
   double* adj_acc = nullptr;      // [4]: dJ/du_in (3), J
   // field statistics
   uint32_t stat_tiles = 0;          // blocks per z plane
-  double* stat_partial = nullptr;   // [POL_NSTAT][nz * stat_tiles]
+  double* stat_partial = nullptr;   // [nz * stat_tiles][POL_NSTAT]
   double* stat_zpart = nullptr;     // [nz * stat_tiles]
   double* stat_dev = nullptr;       // [POL_NSTAT + nz]
   double* stat_host = nullptr;      // pinned copy of stat_dev (one transfer per call)

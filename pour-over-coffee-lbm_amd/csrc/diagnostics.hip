@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(BLOCK, (sizeof(R) == 4 && !EXTRA) ? 8 : 1) k_f
       }
       default: r = stat_identity(s);
     }
-    a.partial[(size_t)s * (a.tiles * a.nz) + blk] = r;  // statistic-major: the fold reads rows
+    a.partial[blk * POL_NSTAT + s] = r;  // the block's statistics in one run of whole lines
   } else if (threadIdx.x == POL_NSTAT) {
     a.zpart[blk] = sh[0][POL_NSTAT] + sh[1][POL_NSTAT];
   }
@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(256) k_stats_final(const double* __restrict__ 
   const int s = stat ? (int)b : 0;
   double r = stat ? stat_identity(s) : 0.0;
   if (stat) {
-    for (uint32_t k = threadIdx.x; k < nblk; k += 256) r = stat_combine(s, r, partial[(size_t)s * nblk + k]);
+    for (uint32_t k = threadIdx.x; k < nblk; k += 256) r = stat_combine(s, r, partial[(size_t)k * POL_NSTAT + s]);
   } else {
     const uint32_t z = b - POL_NSTAT;
     for (uint32_t k = threadIdx.x; k < tiles; k += 256) r += zpart[(size_t)z * tiles + k];

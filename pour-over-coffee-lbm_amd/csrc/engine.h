@@ -88,7 +88,7 @@ struct StatsArgs {
   FastDiv dnx;
   uint32_t tiles;         // blocks per z plane
   int32_t z_top, z_mid, z_out;
-  double* partial;        // [POL_NSTAT][nz * tiles]
+  double* partial;        // [nz * tiles][POL_NSTAT]
   double* zpart;          // [nz * tiles]
 };
 // Statistics into out_dev[POL_NSTAT] and the per-plane density sums into
