@@ -457,8 +457,8 @@ int pol_adjoint_collect(pol_ctx *ctx, double *out4, void *stream);
 #define POL_STAT_INT_CELLS 35     /* non-solid cells with 1 <= x, y, z <= n - 2 */
 #define POL_STAT_INT_WATER 36     /* sum rho over those with phase > 0.1 */
 #define POL_STAT_INT_AIR 37       /* sum rho over the others */
-#define POL_STAT_INT_UMAX 38      /* max |u| over interior non-solid cells; -1e300 when none */
-#define POL_STAT_INT_UMIN 39      /* min |u|; 1e300 when none */
+#define POL_STAT_INT_UMAX 38      /* max |u| over interior non-solid cells (NaN |u| skipped); -1e300 when none */
+#define POL_STAT_INT_UMIN 39      /* min |u| (NaN |u| skipped); 1e300 when none */
 #define POL_STAT_INT_USUM 40      /* sum |u| */
 #define POL_STAT_SUM_UMAG_ACTIVE 41 /* sum |u| over |u| > 1e-10 */
 #define POL_STAT_UMAG_ACTIVE 42   /* cells with |u| > 1e-10 */

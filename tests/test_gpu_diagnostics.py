@@ -210,3 +210,121 @@ def test_full_size_statistics():
     assert abs(s["sum_drho"]) < 1e-3 * s["cells"] and s["fluid_pop_nan"] == 0 and s["fluid_pop_inf"] == 0
     assert abs(zp.sum() - (s["cells"] + s["sum_drho"])) < 1e-6 * s["cells"]
     json.dumps(D.visualizer_statistics(ctx))
+
+
+def _box_ctx(nx, ny, nz, precision="f32"):
+    """A closed box with a few solid cells inside (kind array [z, y, x])."""
+    from pourover_amd.engine import SolverConfig
+    from pourover_amd.geometry import link_flags
+    kind = np.zeros((nz, ny, nx), np.uint8)
+    kind[nz // 2, ny // 2, :] = L.POL_KIND_SOLID
+    kind[0, 0, 0] = L.POL_KIND_SOLID
+    cfg = SolverConfig(nx, ny, nz, precision=precision, tau=0.8, periodic=(False, False, False))
+    ctx = HipContext(cfg, link_flags(kind)[0])
+    solid = (kind == L.POL_KIND_SOLID).transpose(2, 1, 0).astype(np.uint8)
+    return ctx, solid
+
+
+def _expected(rho, u, solid, prev_rho=None, prev_u=None, phase=None):
+    """The statistics of finite fields ([x, y, z] arrays) restated in numpy."""
+    n = rho.size
+    nz = rho.shape[2]
+    r64, m = rho.astype(np.float64), OD.umag(u).astype(np.float64)
+    mach = OD.umag(u) / OD.CS_F32
+    e = {"cells": n, "sum_drho": np.sum(r64 - 1.0), "sum_drho2": np.sum((r64 - 1.0) ** 2),
+         "max_abs_drho": np.max(np.abs(r64 - 1.0)), "sum_umag": m.sum(), "sum_umag2": (m * m).sum(),
+         "max_umag": m.max(), "active": np.sum(mach > np.float32(1e-10)),
+         "mach_gt": np.sum(mach > np.float32(0.1)),
+         "sum_mach_active": mach[mach > np.float32(1e-10)].astype(np.float64).sum(),
+         "umag_active": np.sum(m > np.float32(1e-10)), "sum_umag_active": m[m > np.float32(1e-10)].sum(),
+         "outlet_down": np.maximum(-u[:, :, 0:5, 2].astype(np.float64), 0).sum()}
+    mom = np.sum(r64[..., None] * u.astype(np.float64), axis=(0, 1, 2))
+    e.update(mom_x=mom[0], mom_y=mom[1], mom_z=mom[2])
+    for i, k in enumerate("xyz"):
+        e["sum_u" + k] = u[..., i].astype(np.float64).sum()
+        e["sum_u%s2" % k] = (u[..., i].astype(np.float64) ** 2).sum()
+    zt, zm = int(nz * 0.8), int(nz * 0.3)
+    e.update(sum_umag_top=m[:, :, zt:].sum(), sum_umag_mid=m[:, :, zm:zt].sum(), sum_umag_bot=m[:, :, :zm].sum())
+    e["drho_prev2"] = 0.0 if prev_rho is None else np.sum((r64 - prev_rho.astype(np.float64)) ** 2)
+    e["du_prev2"] = 0.0 if prev_u is None else np.sum((u.astype(np.float64) - prev_u.astype(np.float64)) ** 2)
+    scan = OD.stability_scan(rho, u, solid)
+    e.update(fluid=int(np.sum(solid == 0)), fluid_rho_min=scan["min_density"], fluid_rho_max=scan["max_density"],
+             fluid_umax=scan["max_velocity"])
+    vis = OD.visualizer_statistics(rho, u, solid, np.ones_like(rho) if phase is None else phase)
+    e.update(int_cells=vis["total_nodes"], int_water=vis["total_water_mass"], int_air=vis["total_air_mass"],
+             int_umax=vis["max_velocity"], int_umin=vis["min_velocity"],
+             int_usum=vis["avg_velocity"] * vis["total_nodes"])
+    return e, r64.sum(axis=(0, 1))
+
+
+# (nx, ny, nz): 4-cell groups that never cross a row, groups that cross rows
+# (nx = 6, a plane of 60 cells), one cell per lane (a plane of 323 cells; nx = 3)
+@pytest.mark.parametrize("shape", [(8, 12, 10), (6, 10, 9), (17, 19, 11), (3, 8, 7)])
+@pytest.mark.parametrize("extra", [False, True])
+def test_statistics_kernel_forms(shape, extra):
+    """Every statistic of random finite fields on boxes that take each form of
+    the statistics kernel (four cells per lane with and without row-crossing
+    groups, one cell per lane), with and without a previous state and phase,
+    against the numpy restatement."""
+    print("case: statistics kernel forms %s extra=%s" % (shape, extra), flush=True)
+    ctx, solid = _box_ctx(*shape)
+    rng = np.random.default_rng(sum(shape) + extra)
+    nx, ny, nz = shape
+    r = rng.uniform(0.9, 1.1, (nz, ny, nx)).astype(np.float32)
+    uu = rng.normal(0, 0.05, (3, nz, ny, nx)).astype(np.float32)
+    uu[:, 1, 1, 1] = 0.0                     # |u| = +0: inactive
+    uu[:, 2, 2, 1] = np.float32(1e-12)       # under both activity thresholds
+    ctx.rho.copy_(torch.from_numpy(r))
+    ctx.u.copy_(torch.from_numpy(uu))
+    prev_rho = prev_u = phase = None
+    if extra:
+        prev_rho = ctx.rho * 0.999 + 0.001
+        prev_u = ctx.u * 0.9
+        phase = torch.from_numpy(rng.uniform(-0.5, 1.0, (nz, ny, nx)).astype(np.float32)).cuda()
+    s, zp = D.field_statistics(ctx, prev_rho, prev_u, phase)
+    rho, u = _xyz(ctx.rho), _uxyz(ctx.u)
+    e, ze = _expected(rho, u, solid, None if prev_rho is None else _xyz(prev_rho),
+                      None if prev_u is None else _uxyz(prev_u), None if phase is None else _xyz(phase))
+    exact = {"cells", "max_abs_drho", "max_umag", "active", "mach_gt", "umag_active", "fluid", "fluid_rho_min",
+             "fluid_rho_max", "fluid_umax", "int_cells", "int_umax", "int_umin"}
+    for k, v in e.items():
+        if k in exact:
+            assert s[k] == v, (k, s[k], v)
+        else:
+            assert _close(s[k], v, at=1e-12 * rho.size), (k, s[k], v)
+    assert np.allclose(zp, ze, rtol=RT, atol=0)
+    assert (s["fluid_rho_nan"], s["fluid_rho_inf"], s["fluid_u_nan"], s["fluid_u_inf"]) == (0, 0, 0, 0)
+
+
+def test_statistics_nonfinite_velocity():
+    """|u| extremes and counts with NaN and infinite velocities: max |u| is
+    NaN when any |u| is; the non-solid maximum keeps finite values only; the
+    interior extremes skip NaN (they keep +inf) and give their identities
+    when every interior value is NaN."""
+    print("case: statistics with NaN / inf velocities", flush=True)
+    nx, ny, nz = 8, 8, 8
+    ctx, solid = _box_ctx(nx, ny, nz)
+    base = np.full((3, nz, ny, nx), 0.01, np.float32)
+    base[0, 3, 3, 3] = 0.2                   # the largest finite |u|, interior
+    base[1, 0, 4, 4] = 0.3                   # larger, on the z = 0 face (not interior)
+    ctx.rho.fill_(1.0)
+    ctx.u.copy_(torch.from_numpy(base))
+    u = base.copy()
+    u[2, 2, 5, 5] = np.nan                   # interior, non-solid
+    u[0, 6, 2, 2] = -np.inf                  # interior, non-solid: |u| = +inf
+    u[1, 7, 7, 7] = np.nan                   # boundary face
+    ctx.u.copy_(torch.from_numpy(u))
+    s, _ = D.field_statistics(ctx)
+    assert np.isnan(s["max_umag"]) and np.isnan(s["sum_umag"])
+    assert (s["fluid_u_nan"], s["fluid_u_inf"]) == (2, 1)
+    assert s["fluid_umax"] == float(OD.umag(base[:, 0, 4, 4][None, :])[0])
+    assert s["int_umax"] == np.inf
+    assert s["int_umin"] == float(OD.umag(np.full((1, 3), 0.01, np.float32))[0])
+    # every interior cell NaN: the interior extremes keep their identities
+    u2 = base.copy()
+    u2[0, 1:nz - 1, 1:ny - 1, 1:nx - 1] = np.nan
+    ctx.u.copy_(torch.from_numpy(u2))
+    s, _ = D.field_statistics(ctx)
+    assert s["int_umax"] == -1e300 and s["int_umin"] == 1e300
+    assert s["int_cells"] == int(np.sum(solid[1:nx - 1, 1:ny - 1, 1:nz - 1] == 0))
+    assert s["fluid_umax"] == float(OD.umag(base[:, 0, 4, 4][None, :])[0])
