@@ -23,15 +23,21 @@ WRITE_GRANULE = 128  # bytes: HBM writes narrower than this line cost a read-mod
 #     16-bit 0.114 / 0.098 / 0.110, FP64 0.255 (32)
 #   compact, runs packed (align 1): FP32 0.166 / 0.154 / 0.141,
 #     16-bit 0.091 / 0.093 / 0.100, FP64 0.294 (16)
-#   compact, (chunk, align) = (8, 8) / (16, 16) / (32, 32):
+#   compact, (chunk, align) = (8, 8) / (16, 16) / (32, 32), first measured:
 #     FP32 0.125 / 0.124 / 0.127, FP16 0.081 / 0.082 / 0.087,
 #     BF16 - / 0.090 / -, FP64 0.239 / 0.249 / -
+#   and again with the current kernels (two runs each; the collisionless
+#   pass in brackets; (16, 8) and (32, 16) slower for every storage):
+#     FP32 0.1151, 0.1128 (0.1118) / 0.1169, 0.1150 (0.1147) / 0.1204, 0.1185
+#     FP16 0.0642, 0.0654 (0.0610) / 0.0670, 0.0673 (0.0621) / 0.0749, 0.0738
+#     BF16 0.0666, 0.0653 (0.0618) / 0.0689, 0.0674 (0.0626) / 0.0771, 0.0744
 # Aligned runs write whole aligned segments and leave 2-6 % padding where
-# the dense chunks leave 23 % solid lanes.
+# the dense chunks leave 23 % solid lanes; 8-cell runs pad least (3 845 120
+# stored slots per plane against 3 962 688 for 16 and 4 194 624 for 32).
 COMPACT_DEFAULT = {"f32": True, "f16": True, "bf16": True, "f64": True}
 CHUNK_CELLS = {"f32": 32, "f16": 32, "bf16": 32, "f64": 16}
-CHUNK_CELLS_COMPACT = {"f32": 16, "f16": 8, "bf16": 16, "f64": 8}
-COMPACT_ALIGN = {"f32": 16, "f16": 8, "bf16": 16, "f64": 8}
+CHUNK_CELLS_COMPACT = {"f32": 8, "f16": 8, "bf16": 8, "f64": 8}
+COMPACT_ALIGN = {"f32": 8, "f16": 8, "bf16": 8, "f64": 8}
 _STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 
 
