@@ -123,6 +123,7 @@ class AdjointProblem:
                 red = ctx.step_from(seg[n], out, what)
                 if what:
                     J += float(red[7])
+            ctx.set_current(1 - ctx.current)  # the final state is the context's current one
             return J
         K = self.checkpoint
         last = self.steps - K  # first step of the last segment
@@ -143,6 +144,7 @@ class AdjointProblem:
                     J += float(red[7])
                 if n + 1 == self.steps:
                     self._seg_holds = self.steps // K - 1
+                    ctx.set_current(1 - ctx.current)  # the final state is the context's current one
                 continue
             if self.objective_step(n):
                 red = ctx.step(1, L.POL_WANT_REDUCE)

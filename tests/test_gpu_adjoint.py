@@ -89,14 +89,26 @@ def test_adjoint_f32_close_to_f64(hip):
 
 def test_store_all_equals_checkpointing(hip):
     """checkpoint=None keeps every forward state; the reverse sweep then
-    recomputes nothing and the gradient is the checkpointed one, bit for bit."""
+    recomputes nothing and the gradient is the checkpointed one, bit for bit.
+    The checkpointed forward run leaves its last segment in the segment store
+    (the sweep rebuilds only the others); every forward form ends with the
+    final state as the context's current one, equal to a plain run's."""
     pc, K = _problem(48, "f64", 60, 20, 15)
     ps, _ = _problem(48, "f64", 60, 20, None)
     Jc, gc, uc = pc.gradient()
+    final_c = pc.ctx.f_current.clone()
     Js, gs, us = ps.gradient()
+    final_s = ps.ctx.f_current.clone()
     assert Jc == Js
     assert torch.equal(gc, gs)
     assert np.array_equal(uc, us)
+    pc.forward(checkpoints=False)
+    assert torch.equal(pc.ctx.f_current, final_c) and torch.equal(final_s, final_c)
+    # the last segment the forward run stored is the one the sweep would rebuild
+    pc.forward(checkpoints=True)
+    held = pc.seg.clone()
+    pc._rebuild(pc.steps // pc.checkpoint - 1, pc.seg)
+    assert torch.equal(pc.seg, held)
 
 
 def test_compact_layout_adjoint_equals_dense(hip):
