@@ -13,9 +13,10 @@
 // 224^3.  Here one pass reads each field once on the device.
 //
 // Statistics kernel: a grid of (tiles, nz) blocks, block (t, z) looping over
-// a strip of plane z; per-lane FP64 accumulators (32-bit counters for the
-// counts) split between two wave pairs, wave64 shuffles, one partial per
-// block per statistic, folded in block order by one block (deterministic).
+// a strip of plane z; per-lane FP64 accumulators (integer counters for the
+// counts, integer keys for the extremes of |u|) split between two wave
+// pairs, wave64 shuffles, one partial per block per statistic, folded in
+// block order by one block per statistic (deterministic).
 // The per-plane density sums of the z profile fall out of the same partials.
 // |u| is formed in FP32 as sqrt((ux*ux + uy*uy) + uz*uz) with explicit
 // round-to-nearest operations (no contraction), the value the reference's
@@ -72,7 +73,8 @@ __device__ inline double umag_f32(double ux, double uy, double uz) {
   return umag_f32((float)ux, (float)uy, (float)uz);
 }
 
-// Count statistics accumulate as 32-bit integers (fewer registers than FP64).
+// Count statistics accumulate as integers (two counts in the 16-bit halves of
+// one register per lane; reduced across the wave as 32-bit integers).
 __host__ __device__ constexpr bool stat_is_count(int s) {
   return s == POL_STAT_CELLS || s == POL_STAT_ACTIVE || s == POL_STAT_MACH_GT || s == POL_STAT_FLUID ||
          s == POL_STAT_FLUID_RHO_NAN || s == POL_STAT_FLUID_RHO_INF || s == POL_STAT_FLUID_U_NAN ||
@@ -89,8 +91,8 @@ __host__ __device__ constexpr bool stat_is_count(int s) {
 // formed once per cell, and each wave reduces only its own statistics; the
 // second read of a cell comes from the cache.  The three regional |u| sums
 // are the block's |u| sum placed by its (block-uniform) plane.  When a plane
-// holds a multiple of four cells, lanes read four consecutive cells per load
-// (16-byte loads of rho, u and the flags).
+// holds a multiple of four cells and a row at least four, lanes read four
+// consecutive cells per load (16-byte loads of rho, u and the flags).
 enum StatRole { ROLE_A = 0, ROLE_B = 1, ROLE_REGION = 2, ROLE_NONE = 3 };
 __host__ __device__ constexpr int stat_role(int s) {
   return (s == POL_STAT_SUM_UMAG_TOP || s == POL_STAT_SUM_UMAG_MID || s == POL_STAT_SUM_UMAG_BOT) ? ROLE_REGION
