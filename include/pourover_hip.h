@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define POL_ABI_VERSION 3
+#define POL_ABI_VERSION 4
 #define POL_Q 19
 #define POL_NRED 12 /* length of the reduction vector, see POL_RED_* */
 
@@ -124,13 +124,16 @@ typedef struct pol_buffers {
   void *perm;           /* real [cells] permeability (POL_PERM_FIELD) */
   void *rho;            /* real [cells] macroscopic output, may be NULL */
   void *u;              /* real [3][cells] macroscopic output, may be NULL */
-  const uint32_t *active_chunks; /* first-cell indices of the chunk_cells-cell
-                                    row chunks (x0 % chunk_cells == 0) that
-                                    hold at least one non-solid cell, or NULL
-                                    to launch over every cell.  With a list
-                                    the kernels touch only those chunks; their
-                                    solid lanes store zero deviations (f = w),
-                                    so written lines are whole */
+  const uint32_t *active_chunks; /* dense layout: first-cell indices of the
+                                    chunk_cells-cell row chunks (x0 %
+                                    chunk_cells == 0) that hold at least one
+                                    non-solid cell, or NULL to launch over
+                                    every cell.  With a list the kernels touch
+                                    only those chunks; their solid lanes store
+                                    zero deviations (f = w), so written lines
+                                    are whole.  Compact layout (required
+                                    there): the first-cell indices of the
+                                    launched row pieces (below) */
   uint32_t n_chunks;
   uint32_t chunk_cells; /* cells per chunk: 8, 16, 32 or 64 (0 = POL_CHUNK) */
   void *particle_drag; /* real [4][cells] (POL_PARTICLE_FIELD): per cell
@@ -146,15 +149,18 @@ typedef struct pol_buffers {
    * follow one another in (z, y) order.  Slots 0, 1, stored_cells - 2 and
    * stored_cells - 1 hold no cell (guards: the kernels clamp a row offset
    * into [1, stored_cells - 2] and reach x +- 1 by constant offsets).
-   * Launches then use one record of POL_RECORD_WORDS uint32 per
-   * chunk_cells-cell row piece meeting a run:
-   *   [0] dense index of the chunk's first cell, [1 + a + 3 b] row_base of
+   * Launches go over the chunk_cells-cell row pieces listed in
+   * active_chunks (n_chunks first-cell indices) and read, for a piece's row
+   * (z ny + y = first cell / nx), its record of POL_RECORD_WORDS uint32 in
+   * chunk_records, which holds one record per dense row (nz * ny records,
+   * 16-byte aligned; rows without a run are never read):
+   *   [0] dense index of the row's first cell, [1 + a + 3 b] row_base of
    *   row (y + a - 1, z + b - 1) (a, b in 0..2; a row past a non-periodic
-   *   face is the chunk's own row, periodic faces wrap), [10] own row's run
-   *   x_begin | x_end << 16, [11] the chunk's row y | z << 16 (so nx, ny,
+   *   face is the row itself, periodic faces wrap), [10] the row's run
+   *   x_begin | x_end << 16, [11] the row's y | z << 16 (so nx, ny,
    *   nz < 65536).
    * Every stored slot is written each step (solid cells of a run with zero
-   * deviations).  active_chunks must be NULL with records. */
+   * deviations). */
   const uint32_t *chunk_records;
   uint32_t stored_cells; /* plane stride of the compact layout */
   uint32_t layout_flags; /* POL_LAYOUT_* */
@@ -168,7 +174,7 @@ typedef struct pol_buffers {
  * when true (otherwise x-face cells read wrong, in-bounds sources). */
 #define POL_LAYOUT_NO_X_FACE 1u
 #define POL_CHUNK 32 /* default chunk_cells */
-#define POL_RECORD_WORDS 12 /* uint32 per compact-layout chunk record */
+#define POL_RECORD_WORDS 12 /* uint32 per compact-layout row record */
 
 /* ---- library ------------------------------------------------------------ */
 /* ABI version (POL_ABI_VERSION). */

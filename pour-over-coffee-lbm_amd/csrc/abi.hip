@@ -205,9 +205,10 @@ static int bind_buffers(pol_ctx* c, const pol_buffers* b, bool pair) {
     return fail(c, POL_ERR_INVALID, "pol_bind: chunk_cells must be 8, 16, 32 or 64");
   const uint32_t cshift = cc == 8 ? 3 : (cc == 16 ? 4 : (cc == 32 ? 5 : 6));
   const uint64_t max_chunks = (uint64_t)c->d.ny * c->d.nz * (((uint64_t)c->d.nx + cc - 1) / cc);
-  const bool listed = b->active_chunks || b->chunk_records;
-  if (b->active_chunks && b->chunk_records)
-    return fail(c, POL_ERR_INVALID, "pol_bind: pass either active_chunks (dense layout) or chunk_records (compact)");
+  // the dense layout may list chunks; the compact layout lists its pieces
+  const bool listed = b->active_chunks != nullptr;
+  if (b->chunk_records && !b->active_chunks)
+    return fail(c, POL_ERR_INVALID, "pol_bind: the compact layout needs its piece list in active_chunks");
   if (listed && (b->n_chunks == 0 || b->n_chunks > max_chunks || ((uint64_t)b->n_chunks << cshift) >= (1ull << 32)))
     return fail(c, POL_ERR_INVALID, "pol_bind: n_chunks out of range");
   if (listed && !(o & POL_USE_FLAGS))
@@ -298,7 +299,7 @@ static StepArgs make_args(const pol_ctx* c, const void* fin, void* fout, uint32_
   a.chunks = c->b.active_chunks;
   a.records = c->b.chunk_records;
   a.P = c->P;
-  a.n_items = (c->b.active_chunks || c->b.chunk_records) ? c->b.n_chunks << c->cshift : c->N;
+  a.n_items = c->b.active_chunks ? c->b.n_chunks << c->cshift : c->N;
   a.cshift = c->cshift;
   a.xface = (c->b.layout_flags & POL_LAYOUT_NO_X_FACE) ? 0u : 1u;
   // Streaming (non-temporal) stores once a step's population traffic (read +
@@ -503,7 +504,7 @@ int pol_reduce_noalloc(const pol_desc* desc, const pol_buffers* bufs, double* re
   c.grid = make_grid(N);
   c.wgrid = c.grid;
   c.part_cap = POL_NOALLOC_MAX_BLOCKS;
-  const bool listed = bufs->active_chunks || bufs->chunk_records;
+  const bool listed = bufs->active_chunks != nullptr;
   const uint32_t cc = bufs->chunk_cells ? bufs->chunk_cells : POL_CHUNK;
   const uint64_t items = listed ? (uint64_t)bufs->n_chunks * cc : N;
   if ((items + BLOCK - 1) / BLOCK > POL_NOALLOC_MAX_BLOCKS)

@@ -121,8 +121,8 @@ __global__ void __launch_bounds__(BLOCK) k_adjoint(const AdjointArgs A, const ui
   bool in_run = false;
   bool inside;
   if constexpr (SPARSE) {
-    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.nx, a.ny, cell, y, z, x,
-                         rb, sx, in_run);
+    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.records, a.cshift, a.N, a.dnx, a.nx,
+                         cell, y, z, x, rb, sx, in_run);
   } else {
     uint32_t yz;
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);

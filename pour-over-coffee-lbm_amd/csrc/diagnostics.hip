@@ -488,8 +488,8 @@ __device__ inline bool work_cell(const StepArgs& a, uint32_t lblk, uint32_t& cel
   int sx = 0;
   bool in_run = false, inside;
   if constexpr (SPARSE)
-    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.records, a.cshift, a.N, a.nx, a.ny, cell, y, z, x, rb,
-                         sx, in_run);
+    inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.records, a.cshift, a.N, a.dnx, a.nx,
+                         cell, y, z, x, rb, sx, in_run);
   else
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
   own = cell;

@@ -29,8 +29,8 @@ struct StepArgs {
   void* rho;          // real [N]
   void* u;            // real [3][N]
   double* partial;    // [nblk][POL_NRED]
-  const uint32_t* chunks;  // chunk list: item -> chunks[item >> cshift] + lane (null: item = cell)
-  const uint32_t* records; // compact layout: chunk records (POL_RECORD_WORDS each), else null
+  const uint32_t* chunks;  // chunk / piece list: item -> chunks[item >> cshift] + lane (null: item = cell)
+  const uint32_t* records; // compact layout: one row record (POL_RECORD_WORDS) per dense row, else null
   uint32_t P;              // population plane stride: N (dense) or the compact layout's stored cells
   uint32_t n_items;        // number of work items (cells, or chunk cells x chunks)
   uint32_t cshift;         // log2 of the cells per chunk

@@ -263,41 +263,43 @@ __device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t
 }
 
 // Work item -> cell for the compact layout: lane (item & (2^cshift - 1)) of
-// the chunk described by records[item >> cshift] (POL_RECORD_WORDS words:
-// dense first cell, the nine neighbour-row bases, the own row's stored run,
-// the row's y | z << 16).
+// the piece whose first dense cell is pieces[item >> cshift]; its row
+// (z ny + y, one division by nx) selects the row record
+// records[row * POL_RECORD_WORDS] (the row's first cell, the nine
+// neighbour-row bases, the row's stored run, y | z << 16).  The cell index,
+// x and the validity come from the piece alone, so a caller's load of the
+// cell's flags goes out beside the row-record load.
 // rb[i] is the slot base of row (y + i%3 - 1, z + i/3 - 1).  sx = x0 + lane
-// is the lane's x counted from its chunk's row without wrapping (a stored run
+// is the lane's x counted from its piece's row without wrapping (a stored run
 // may extend past the row end as padding), and in_run tells whether sx lies
 // in the own row's stored run, i.e. whether the lane owns slot rb[4] + sx.
-__device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ records,
-                                   uint32_t cshift, uint32_t N, uint32_t nx, uint32_t ny, uint32_t& cell,
-                                   uint32_t& y, uint32_t& z, uint32_t& x, int (&rb)[9], int& sx, bool& in_run) {
+__device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ pieces,
+                                   const uint32_t* __restrict__ records, uint32_t cshift, uint32_t N,
+                                   const FastDiv& dnx, uint32_t nx, uint32_t& cell, uint32_t& y, uint32_t& z,
+                                   uint32_t& x, int (&rb)[9], int& sx, bool& in_run) {
   const bool listed = item < n_items;
-  bool ok = listed;
   const uint32_t lane = item & ((1u << cshift) - 1u);
+  const uint32_t first = listed ? pieces[item >> cshift] : 0u;
+  const uint32_t row = fdiv(first, dnx);  // z * ny + y
+  const uint32_t x0 = first - row * nx;
+  sx = (int)(x0 + lane);
+  const uint32_t cc = first + lane;
+  const bool ok = listed && x0 + lane < nx && cc < N;  // lanes past the row end are padding
+  cell = ok ? cc : 0u;
+  x = ok ? x0 + lane : 0u;
   uint32_t w[12] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-  if (listed) {
-    const uint4* r = (const uint4*)(records + (size_t)(item >> cshift) * POL_RECORD_WORDS);
+  if (listed && first < N) {  // row < ny nz: inside the record table
+    const uint4* r = (const uint4*)(records + (size_t)row * POL_RECORD_WORDS);
     const uint4 a = r[0], b = r[1], c = r[2];
     w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
     w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
     w[8] = c.x; w[9] = c.y; w[10] = c.z; w[11] = c.w;
   }
-  // the chunk's row (y, z) from word 11: no division per lane
-  const uint32_t y0 = w[11] & 0xffffu, z0 = w[11] >> 16;
-  const uint32_t x0 = w[0] - (z0 * ny + y0) * nx;
-  sx = (int)(x0 + lane);
-  x = x0 + lane;
-  const uint32_t cc = w[0] + lane;
-  ok = ok && x < nx && cc < N;  // lanes past the row end are padding
 #pragma unroll
   for (int i = 0; i < 9; ++i) rb[i] = (int)w[1 + i];
-  in_run = listed && sx >= (int)(w[10] & 0xffffu) && sx < (int)(w[10] >> 16);
-  cell = ok ? cc : 0u;
-  y = ok ? y0 : 0u;
-  z = ok ? z0 : 0u;
-  if (!ok) x = 0;
+  in_run = listed && first < N && sx >= (int)(w[10] & 0xffffu) && sx < (int)(w[10] >> 16);
+  y = ok ? (w[11] & 0xffffu) : 0u;
+  z = ok ? (w[11] >> 16) : 0u;
   return ok;
 }
 

@@ -192,6 +192,7 @@ class HipContext:
         b.u = self.u.data_ptr()
         if self.records is not None:
             b.chunk_records = self.records.data_ptr()
+            b.active_chunks = self.chunks.data_ptr()  # the launched pieces
             b.n_chunks = self.layout.n_chunks
             b.stored_cells = self.layout.stored_cells
             b.layout_flags = 0 if self.layout.x_face else L.POL_LAYOUT_NO_X_FACE
@@ -207,14 +208,14 @@ class HipContext:
 
     def _set_layout(self, flags: Optional[np.ndarray]) -> None:
         """Launch list and population layout for these flags: the compact
-        row-run layout with chunk records, or the dense layout with (when
-        flags are given) a dense chunk list."""
+        row-run layout (its pieces and row records), or the dense layout with
+        (when flags are given) a dense chunk list."""
         cfg = self.config
         if self.compact:
             self.layout = build_compact_layout(flags, tuple(bool(p) for p in cfg.periodic), self.chunk_cells,
                                                self.align)
-            self.records = _as_int32_view(self.layout.records.reshape(-1)).to(self.device)
-            self.chunks = None
+            self.records = _as_int32_view(self.layout.row_records.reshape(-1)).to(self.device)
+            self.chunks = _as_int32_view(self.layout.pieces).to(self.device)
             self.plane = self.layout.stored_cells
         else:
             self.layout = None

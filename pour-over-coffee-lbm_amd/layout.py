@@ -19,21 +19,26 @@ M - 1 free of cells: a kernel may then clamp a row offset (source slot + e_x)
 into [1, M - 2] once and reach the x - 1 / x + 1 neighbours with constant
 offsets, without the clamp ever moving a cell's own source.
 
-A launch uses chunk records instead of the dense chunk list: one record of
-RECORD_WORDS uint32 per 32-cell row chunk that meets a run,
-    [0]     dense index of the chunk's first cell (x0 + nx (y + ny z))
+A launch lists its row pieces (`pieces`: the dense index x0 + nx (y + ny z)
+of each piece's first cell, one uint32 per chunk of chunk_cells cells that
+meets a run) and reads one row record of RECORD_WORDS uint32 per dense row
+(`row_records[z * ny + y]`, the row found from the piece's first cell by one
+division by nx):
+    [0]     dense index of the row's first cell (nx (y + ny z))
     [1..9]  row_base of the rows (y + a - 1, z + b - 1), a, b in {0, 1, 2},
             at word 1 + a + 3 b; rows outside a non-periodic face resolve to
-            the chunk's own row (the clamp the gather applies), periodic
-            faces wrap
-    [10]    own row's run: x_begin | x_end << 16
-    [11]    the chunk's row: y | z << 16 (the kernels take (x, y, z) from it
-            without a division)
-so a thread finds every neighbour slot from its record without further
+            the row itself (the clamp the gather applies), periodic faces
+            wrap
+    [10]    the row's run: x_begin | x_end << 16
+    [11]    the row: y | z << 16
+so a thread finds every neighbour slot from its row record without further
 lookups: source of link q = word[1 + (1 - e_y) + 3 (1 - e_z)] + x - e_x
-(x wrapped or clamped like the dense gather).  With align > 1 each row's
-storage starts on an aligned slot and is padded to a multiple of align, and
-the chunks are run pieces starting at the run's first cell (see
+(x wrapped or clamped like the dense gather).  The cell index and its x come
+from the piece alone, so the cell's flags and the row record load side by
+side.  Pieces of one row share its record: 4 B per piece plus 48 B per row
+(about 4 MB at 224^3) instead of a 48-byte record per piece (23 MB).  With
+align > 1 each row's storage starts on an aligned slot and is padded to a
+multiple of align, and the pieces start at the run's first cell (see
 build_compact_layout).
 """
 from __future__ import annotations
@@ -57,14 +62,23 @@ class CompactLayout:
     used_cells: int          # leading guard + slots of the stored runs (padding included)
     row_base: np.ndarray     # int64 [nz*ny]
     row_run: np.ndarray      # uint32 [nz*ny]: x_begin | x_end << 16 (0 = empty row); x_end may pass nx (padding)
-    records: np.ndarray      # uint32 [n_chunks, RECORD_WORDS]
+    pieces: np.ndarray       # uint32 [n_chunks]: dense index of each launched piece's first cell
+    row_records: np.ndarray  # uint32 [nz*ny, RECORD_WORDS]
     slot_dense: np.ndarray   # int64 [used_cells]: dense cell of slot s, -1 for guard and padding
     align: int = 1           # slots a row's storage is aligned and padded to
     x_face: bool = True      # some run holds a cell at x = 0 or x = nx - 1
 
     @property
     def n_chunks(self) -> int:
-        return int(self.records.shape[0])
+        return int(self.pieces.shape[0])
+
+    @property
+    def records(self) -> np.ndarray:
+        """Per-piece view [n_chunks, RECORD_WORDS] of what a kernel decodes for
+        a piece: word 0 the piece's first cell, words 1..11 its row record."""
+        rec = self.row_records[self.pieces.astype(np.int64) // self.nx].copy()
+        rec[:, 0] = self.pieces
+        return rec
 
     @property
     def dense_index(self) -> np.ndarray:
@@ -98,7 +112,8 @@ class CompactLayout:
 
 def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chunk_cells: int = 32,
                          align: int = 1) -> CompactLayout:
-    """Row runs, slot map and chunk records of a cell-flag array [nz, ny, nx].
+    """Row runs, slot map, launched pieces and row records of a cell-flag
+    array [nz, ny, nx].
 
     align = 1 packs the runs back to back and launches the chunk_cells-cell
     row chunks (x0 a multiple of chunk_cells) that meet a run.  align > 1
@@ -152,9 +167,14 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
         rr = np.repeat(np.arange(nz * ny, dtype=np.int64), pieces)
         k = np.arange(int(pieces.sum()), dtype=np.int64) - np.repeat(np.cumsum(pieces) - pieces, pieces)
         first = rr * nx + xb[rr] + k * chunk_cells
-    zz, yy = rr // ny, rr % ny
-    rec = np.zeros((rr.size, RECORD_WORDS), dtype=np.int64)
-    rec[:, 0] = first
+    if nx * ny * nz >= 1 << 32:
+        raise ValueError("dense cell indices must fit 32 bits")
+    pieces = first.astype(np.uint32)
+    # one record per dense row (rows without a run keep theirs unreferenced)
+    ra = np.arange(nz * ny, dtype=np.int64)
+    zz, yy = ra // ny, ra % ny
+    rec = np.zeros((nz * ny, RECORD_WORDS), dtype=np.int64)
+    rec[:, 0] = ra * nx
     py, pz = bool(periodic[1]), bool(periodic[2])
     for b in range(3):
         z2 = zz + (b - 1)
@@ -162,9 +182,9 @@ def build_compact_layout(flags: np.ndarray, periodic=(False, False, False), chun
         for a in range(3):
             y2 = yy + (a - 1)
             y2 = np.where((y2 < 0) | (y2 >= ny), (y2 % ny) if py else yy, y2)
-            rec[:, 1 + a + 3 * b] = row_base[z2 * ny + y2]
-    rec[:, 10] = row_run[rr]
+            rec[:, 1 + a + 3 * b] = np.where(has, row_base[z2 * ny + y2], 0)
+    rec[:, 10] = row_run
     rec[:, 11] = yy | (zz << 16)
-    records = (rec & 0xFFFFFFFF).astype(np.uint32)  # row bases as two's-complement int32
+    row_records = (rec & 0xFFFFFFFF).astype(np.uint32)  # row bases as two's-complement int32
     x_face = bool((has & ((xb == 0) | (xe == nx))).any())
-    return CompactLayout(nx, ny, nz, M, used, row_base, row_run, records, slot_dense, align, x_face)
+    return CompactLayout(nx, ny, nz, M, used, row_base, row_run, pieces, row_records, slot_dense, align, x_face)

@@ -13,17 +13,21 @@ from pourover_amd.layout import RECORD_WORDS, build_compact_layout
 
 
 def _lanes(lay, chunk_cells=32, with_padding=False):
-    """(record row, x, y, z) of every lane of every chunk; lanes past the row
-    end dropped (the kernel's wrap test), or with with_padding also returned
-    with their unwrapped x (record row, sx, y, z, valid)."""
-    rec = lay.records
-    base = rec[:, 0].astype(np.int64)
-    # the kernels' decode (lbm_common.h item_record): the row from word 11,
-    # x0 = first - row nx, lanes at x0 + lane, valid below nx
+    """(piece, x, y, z) of every lane of every piece; lanes past the row end
+    dropped (the kernel's wrap test), or with with_padding also returned
+    with their unwrapped x (piece, sx, y, z, valid)."""
+    # the kernels' decode (lbm_common.h item_record): the piece's first cell,
+    # its row = first // nx selects the row record, x0 = first - row nx,
+    # lanes at x0 + lane, valid below nx; y, z from the record's word 11
+    base = lay.pieces.astype(np.int64)
+    row = base // lay.nx
+    rec = lay.row_records[row]
+    assert np.array_equal(lay.records[:, 1:], rec[:, 1:]) and np.array_equal(lay.records[:, 0], lay.pieces)
     y0 = (rec[:, 11] & 0xFFFF).astype(np.int64)
     z0 = (rec[:, 11] >> 16).astype(np.int64)
-    assert np.array_equal(z0 * lay.ny + y0, base // lay.nx)  # word 11 is the first cell's row
-    x0 = base - (z0 * lay.ny + y0) * lay.nx
+    assert np.array_equal(z0 * lay.ny + y0, row)  # word 11 is the row's (y, z)
+    assert np.array_equal(rec[:, 0].astype(np.int64), row * lay.nx)  # word 0 its first cell
+    x0 = base - row * lay.nx
     lane = np.arange(chunk_cells, dtype=np.int64)
     cell = base[:, None] + lane[None, :]
     sx = x0[:, None] + lane[None, :]
@@ -69,7 +73,7 @@ def test_gather_slots_match_dense_sources(name, align, chunk):
     kind, flags, rev, per = _unpack(_geometries()[name])
     nz, ny, nx = kind.shape
     lay = build_compact_layout(flags, per, chunk_cells=chunk, align=align)
-    assert lay.records.shape[1] == RECORD_WORDS
+    assert lay.row_records.shape == (nz * ny, RECORD_WORDS) and lay.pieces.shape == (lay.n_chunks,)
     r, x, y, z = _lanes(lay, chunk)
     k = kind[z, y, x]
     run = lay.records[r, 10]
