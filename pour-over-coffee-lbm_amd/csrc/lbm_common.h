@@ -188,7 +188,9 @@ __device__ inline uint32_t logical_block(uint32_t per_xcd) {
 //   ORDER_XCD_RANGES  logical_block above.
 // Measured on the 224^3 V60 step (compact layout, streaming stores), FP32 /
 // FP16 / FP64 ms: linear 0.1145 / 0.0623 / 0.2135, k = 2 0.1140 / 0.0609 /
-// 0.2141, XCD ranges 0.1185 / 0.0655 / 0.2322.  A step held in the
+// 0.2141, XCD ranges 0.1185 / 0.0655 / 0.2322; with 8-cell pieces and row
+// records, FP32 / FP16: k = 2 0.1112 / 0.0620, linear 0.1119 / 0.0635,
+// k = 1, 3, 4 0.1118-0.1122 / 0.0622-0.0628.  A step held in the
 // last-level cache (128^3) is faster with XCD ranges: neighbour rows then
 // hit the XCD's own L2.
 constexpr uint32_t ORDER_LINEAR = 0;
