@@ -135,7 +135,7 @@ typedef struct pol_buffers {
                                     there): the first-cell indices of the
                                     launched row pieces (below) */
   uint32_t n_chunks;
-  uint32_t chunk_cells; /* cells per chunk: 8, 16, 32 or 64 (0 = POL_CHUNK) */
+  uint32_t chunk_cells; /* cells per chunk: 4, 8, 16, 32 or 64 (0 = POL_CHUNK) */
   void *particle_drag; /* real [4][cells] (POL_PARTICLE_FIELD): per cell
                           sum_p w beta_p v_p (x, y, z) and B = sum_p w beta_p
                           over the particles whose trilinear footprint covers

@@ -201,9 +201,9 @@ static int bind_buffers(pol_ctx* c, const pol_buffers* b, bool pair) {
   if ((o & POL_LES_FD) && !b->nu_sgs) return fail(c, POL_ERR_INVALID, "pol_bind: POL_LES_FD without nu_sgs buffer");
   if ((o & POL_PERM_FIELD) && !(o & POL_USE_FLAGS)) return fail(c, POL_ERR_INVALID, "pol_bind: permeability needs cell flags");
   const uint32_t cc = b->chunk_cells ? b->chunk_cells : POL_CHUNK;
-  if (cc != 8 && cc != 16 && cc != 32 && cc != 64)
-    return fail(c, POL_ERR_INVALID, "pol_bind: chunk_cells must be 8, 16, 32 or 64");
-  const uint32_t cshift = cc == 8 ? 3 : (cc == 16 ? 4 : (cc == 32 ? 5 : 6));
+  if (cc != 4 && cc != 8 && cc != 16 && cc != 32 && cc != 64)
+    return fail(c, POL_ERR_INVALID, "pol_bind: chunk_cells must be 4, 8, 16, 32 or 64");
+  const uint32_t cshift = cc == 4 ? 2 : (cc == 8 ? 3 : (cc == 16 ? 4 : (cc == 32 ? 5 : 6)));
   const uint64_t max_chunks = (uint64_t)c->d.ny * c->d.nz * (((uint64_t)c->d.nx + cc - 1) / cc);
   // the dense layout may list chunks; the compact layout lists its pieces
   const bool listed = b->active_chunks != nullptr;

@@ -31,13 +31,20 @@ WRITE_GRANULE = 128  # bytes: HBM writes narrower than this line cost a read-mod
 #     FP32 0.1151, 0.1128 (0.1118) / 0.1169, 0.1150 (0.1147) / 0.1204, 0.1185
 #     FP16 0.0642, 0.0654 (0.0610) / 0.0670, 0.0673 (0.0621) / 0.0749, 0.0738
 #     BF16 0.0666, 0.0653 (0.0618) / 0.0689, 0.0674 (0.0626) / 0.0771, 0.0744
-# Aligned runs write whole aligned segments and leave 2-6 % padding where
-# the dense chunks leave 23 % solid lanes; 8-cell runs pad least (3 845 120
-# stored slots per plane against 3 962 688 for 16 and 4 194 624 for 32).
+#   and with the launched pieces read as first cells plus one record per
+#   row (4 / 2 / 8 cells, chunk = align, two runs each, same box):
+#     FP32 0.1096, 0.1075 (0.1065) / 0.1217, 0.1197 / 0.1120, 0.1102 (0.1090)
+#     FP16 0.0619, 0.0622, 0.0639, 0.0637 / 0.0681, 0.0665 /
+#          0.0627, 0.0637, 0.0624, 0.0651
+#     BF16 0.0641, 0.0639, 0.0642, 0.0648 / 0.0689, 0.0677 /
+#          0.0666, 0.0649, 0.0647, 0.0658
+# Aligned runs write whole aligned segments and leave 1-6 % padding where
+# the dense chunks leave 23 % solid lanes (stored slots per plane at 224^3:
+# 3 783 296 for 4-cell runs, 3 845 120 for 8, 3 962 688 for 16).
 COMPACT_DEFAULT = {"f32": True, "f16": True, "bf16": True, "f64": True}
 CHUNK_CELLS = {"f32": 32, "f16": 32, "bf16": 32, "f64": 16}
-CHUNK_CELLS_COMPACT = {"f32": 8, "f16": 8, "bf16": 8, "f64": 8}
-COMPACT_ALIGN = {"f32": 8, "f16": 8, "bf16": 8, "f64": 8}
+CHUNK_CELLS_COMPACT = {"f32": 4, "f16": 4, "bf16": 4, "f64": 8}
+COMPACT_ALIGN = {"f32": 4, "f16": 4, "bf16": 4, "f64": 8}
 _STORAGE_TORCH = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 
 

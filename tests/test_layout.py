@@ -40,7 +40,7 @@ def _lanes(lay, chunk_cells=32, with_padding=False):
     return r[ok], sx[ok], yb[ok], zb[ok]
 
 
-LAYOUTS = [(1, 32), (8, 8), (32, 32), (16, 16), (32, 64), (64, 64)]
+LAYOUTS = [(1, 32), (4, 4), (8, 8), (32, 32), (16, 16), (32, 64), (64, 64)]
 
 
 def _rb(lay, r, a, b):
