@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define POL_ABI_VERSION 4
+#define POL_ABI_VERSION 5
 #define POL_Q 19
 #define POL_NRED 12 /* length of the reduction vector, see POL_RED_* */
 
@@ -150,7 +150,10 @@ typedef struct pol_buffers {
    * stored_cells - 1 hold no cell (guards: the kernels clamp a row offset
    * into [1, stored_cells - 2] and reach x +- 1 by constant offsets).
    * Launches go over the chunk_cells-cell row pieces listed in
-   * active_chunks (n_chunks first-cell indices) and read, for a piece's row
+   * active_chunks (n_chunks piece words: the first-cell index, plus
+   * POL_PIECE_PLAIN when every cell of the piece with x < nx has the flag
+   * word 0, which the kernels then take without loading it; so
+   * nx ny nz < 2^31) and read, for a piece's row
    * (z ny + y = first cell / nx), its record of POL_RECORD_WORDS uint32 in
    * chunk_records, which holds one record per dense row (nz * ny records,
    * 16-byte aligned; rows without a run are never read):
@@ -175,6 +178,10 @@ typedef struct pol_buffers {
 #define POL_LAYOUT_NO_X_FACE 1u
 #define POL_CHUNK 32 /* default chunk_cells */
 #define POL_RECORD_WORDS 12 /* uint32 per compact-layout row record */
+/* Compact-layout piece word bit: every cell of the piece inside its row has
+ * cell_flags 0 (fluid, no bounce link, not porous, LES on).  Set it only
+ * when true: the kernels use flag word 0 for those cells. */
+#define POL_PIECE_PLAIN 0x80000000u
 
 /* ---- library ------------------------------------------------------------ */
 /* ABI version (POL_ABI_VERSION). */

@@ -16,12 +16,13 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libpourover_hip.so"
 
 # ---- constants (mirror include/pourover_hip.h) ------------------------------
-POL_ABI_VERSION = 4
+POL_ABI_VERSION = 5
 POL_Q = 19
 POL_NRED = 12
 POL_NOALLOC_MAX_BLOCKS = 65536
 POL_CHUNK = 32  # cells per launch chunk
 POL_RECORD_WORDS = 12  # uint32 per compact-layout row record
+POL_PIECE_PLAIN = 0x80000000  # piece word bit: every cell of the piece has flag word 0
 POL_LAYOUT_NO_X_FACE = 1  # pol_buffers.layout_flags: no run reaches x = 0 or nx - 1
 POL_OK, POL_ERR_INVALID, POL_ERR_HIP, POL_ERR_ALLOC, POL_ERR_NO_DEVICE, POL_ERR_STATE, POL_ERR_UNSUPPORTED = range(7)
 POL_F32, POL_F16, POL_BF16, POL_F64 = range(4)

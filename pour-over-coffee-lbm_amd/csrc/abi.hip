@@ -213,6 +213,8 @@ static int bind_buffers(pol_ctx* c, const pol_buffers* b, bool pair) {
     return fail(c, POL_ERR_INVALID, "pol_bind: n_chunks out of range");
   if (listed && !(o & POL_USE_FLAGS))
     return fail(c, POL_ERR_INVALID, "pol_bind: a chunk list needs cell flags (solid lanes are masked by them)");
+  if (b->chunk_records && (uint64_t)c->d.nx * c->d.ny * c->d.nz >= (1ull << 31))
+    return fail(c, POL_ERR_INVALID, "pol_bind: the compact layout needs nx ny nz < 2^31 (piece word bit 31 is POL_PIECE_PLAIN)");
   if (b->chunk_records && (b->stored_cells < 4 || b->stored_cells >= (1u << 31)))
     return fail(c, POL_ERR_INVALID, "pol_bind: compact layout needs 4 <= stored_cells < 2^31");
   if (b->layout_flags & ~POL_LAYOUT_NO_X_FACE)

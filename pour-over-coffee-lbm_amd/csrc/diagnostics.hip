@@ -486,15 +486,15 @@ __device__ inline bool work_cell(const StepArgs& a, uint32_t lblk, uint32_t& cel
   uint32_t yz, x, y, z;
   int rb[9];
   int sx = 0;
-  bool in_run = false, inside;
+  bool in_run = false, plain = false, inside;  // plain: every flag word of the piece is 0
   if constexpr (SPARSE)
     inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.records, a.cshift, a.N, a.dnx, a.nx,
-                         cell, y, z, x, rb, sx, in_run);
+                         cell, y, z, x, rb, sx, in_run, plain);
   else
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
   own = cell;
   if constexpr (SPARSE) own = slot_clamp(rb[4] + sx, a.P);
-  fl = (inside && a.flags) ? a.flags[cell] : (inside ? (POL_KIND_FLUID << POL_KIND_SHIFT) : 0u);
+  fl = (inside && a.flags) ? (plain ? 0u : a.flags[cell]) : (inside ? (POL_KIND_FLUID << POL_KIND_SHIFT) : 0u);
   const uint32_t kind = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;
   return inside && kind != POL_KIND_SOLID;
 }

@@ -270,7 +270,9 @@ __device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t
 // records[row * POL_RECORD_WORDS] (the row's first cell, the nine
 // neighbour-row bases, the row's stored run, y | z << 16).  The cell index,
 // x and the validity come from the piece alone, so a caller's load of the
-// cell's flags goes out beside the row-record load.
+// cell's flags goes out beside the row-record load.  Bit 31 of a piece word
+// marks a plain piece (POL_PIECE_PLAIN: every cell of it inside the row has
+// the flag word 0); plain returns it so that a caller may skip that load.
 // rb[i] is the slot base of row (y + i%3 - 1, z + i/3 - 1).  sx = x0 + lane
 // is the lane's x counted from its piece's row without wrapping (a stored run
 // may extend past the row end as padding), and in_run tells whether sx lies
@@ -278,10 +280,12 @@ __device__ inline bool item_cell(uint32_t item, uint32_t n_items, const uint32_t
 __device__ inline bool item_record(uint32_t item, uint32_t n_items, const uint32_t* __restrict__ pieces,
                                    const uint32_t* __restrict__ records, uint32_t cshift, uint32_t N,
                                    const FastDiv& dnx, uint32_t nx, uint32_t& cell, uint32_t& y, uint32_t& z,
-                                   uint32_t& x, int (&rb)[9], int& sx, bool& in_run) {
+                                   uint32_t& x, int (&rb)[9], int& sx, bool& in_run, bool& plain) {
   const bool listed = item < n_items;
   const uint32_t lane = item & ((1u << cshift) - 1u);
-  const uint32_t first = listed ? pieces[item >> cshift] : 0u;
+  const uint32_t pw = listed ? pieces[item >> cshift] : 0u;
+  const uint32_t first = pw & ~POL_PIECE_PLAIN;
+  plain = (pw & POL_PIECE_PLAIN) != 0u;
   const uint32_t row = fdiv(first, dnx);  // z * ny + y
   const uint32_t x0 = first - row * nx;
   sx = (int)(x0 + lane);

@@ -77,9 +77,9 @@ __global__ void __launch_bounds__(BLOCK) k_init_equilibrium_sparse(const StepArg
   uint32_t cell, x, y, z;
   int rb[9];
   int sx = 0;
-  bool in_run = false;
+  bool in_run = false, plain = false;
   const bool ok = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.records, a.cshift, a.N, a.dnx,
-                              a.nx, cell, y, z, x, rb, sx, in_run);
+                              a.nx, cell, y, z, x, rb, sx, in_run, plain);
   if (!in_run) return;
   const uint32_t N = a.N, P = a.P;
   const uint32_t own = slot_clamp(rb[4] + sx, P);

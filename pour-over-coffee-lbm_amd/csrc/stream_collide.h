@@ -65,10 +65,11 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
   int rb[9];          // compact layout: slot bases of the nine neighbour rows
   int sx = 0;         // compact layout: the lane's unwrapped x (its slot is rb[4] + sx)
   bool in_run = false;
+  bool plain = false;  // compact layout: every flag word of the piece is 0
   bool inside;
   if constexpr (SPARSE) {
     inside = item_record(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.records, a.cshift, a.N, a.dnx, a.nx,
-                         cell, y, z, x, rb, sx, in_run);
+                         cell, y, z, x, rb, sx, in_run, plain);
   } else {
     uint32_t yz;
     inside = item_cell(lblk * BLOCK + threadIdx.x, a.n_items, a.chunks, a.cshift, a.N, a.dnx, a.nx, cell, yz, x);
@@ -76,7 +77,7 @@ __global__ void __launch_bounds__(BLOCK) k_stream_collide(const StepArgs a) {
     y = yz - z * a.ny;
   }
 
-  const uint32_t fl = FLAGS ? (inside ? a.flags[cell] : (POL_KIND_SOLID << POL_KIND_SHIFT)) : 0u;
+  const uint32_t fl = FLAGS ? (inside ? (plain ? 0u : a.flags[cell]) : (POL_KIND_SOLID << POL_KIND_SHIFT)) : 0u;
   const uint32_t kind = (fl >> POL_KIND_SHIFT) & POL_KIND_MASK;
   const bool active = inside && kind != POL_KIND_SOLID;
 

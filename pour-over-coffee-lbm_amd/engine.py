@@ -222,7 +222,7 @@ class HipContext:
             self.layout = build_compact_layout(flags, tuple(bool(p) for p in cfg.periodic), self.chunk_cells,
                                                self.align)
             self.records = _as_int32_view(self.layout.row_records.reshape(-1)).to(self.device)
-            self.chunks = _as_int32_view(self.layout.pieces).to(self.device)
+            self.chunks = _as_int32_view(self.layout.piece_words).to(self.device)
             self.plane = self.layout.stored_cells
         else:
             self.layout = None

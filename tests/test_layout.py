@@ -16,10 +16,14 @@ def _lanes(lay, chunk_cells=32, with_padding=False):
     """(piece, x, y, z) of every lane of every piece; lanes past the row end
     dropped (the kernel's wrap test), or with with_padding also returned
     with their unwrapped x (piece, sx, y, z, valid)."""
-    # the kernels' decode (lbm_common.h item_record): the piece's first cell,
-    # its row = first // nx selects the row record, x0 = first - row nx,
-    # lanes at x0 + lane, valid below nx; y, z from the record's word 11
-    base = lay.pieces.astype(np.int64)
+    # the kernels' decode (lbm_common.h item_record): the piece word's first
+    # cell (bit 31 masked off), its row = first // nx selects the row record,
+    # x0 = first - row nx, lanes at x0 + lane, valid below nx; y, z from the
+    # record's word 11
+    words = lay.piece_words
+    assert words.dtype == np.uint32
+    base = (words & np.uint32(~L.POL_PIECE_PLAIN & 0xFFFFFFFF)).astype(np.int64)
+    assert np.array_equal(base, lay.pieces.astype(np.int64))
     row = base // lay.nx
     rec = lay.row_records[row]
     assert np.array_equal(lay.records[:, 1:], rec[:, 1:]) and np.array_equal(lay.records[:, 0], lay.pieces)
@@ -172,6 +176,31 @@ def test_dense_round_trip_and_every_slot_written(align, chunk):
     if align == 1:
         # packed runs: one slot per run cell
         assert lay.used_cells == 64 + int(((lay.row_run >> 16) - (lay.row_run & 0xFFFF)).astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("align,chunk", LAYOUTS)
+@pytest.mark.parametrize("name", ["v60_48", "periodic_xy"])
+def test_plain_pieces(name, align, chunk):
+    """A piece word carries POL_PIECE_PLAIN exactly when every lane of the
+    piece inside its row has flag word 0: the kernels then take 0 for those
+    lanes without loading the flags, so a set bit with a nonzero flag under
+    it would run a wall or porous cell as plain fluid."""
+    kind, flags, rev, per = _unpack(_geometries()[name])
+    nz, ny, nx = kind.shape
+    lay = build_compact_layout(flags, per, chunk_cells=chunk, align=align)
+    words = lay.piece_words
+    plain = (words & np.uint32(L.POL_PIECE_PLAIN)) != 0
+    assert np.array_equal(plain, lay.plain)
+    lane = np.arange(chunk, dtype=np.int64)
+    first = lay.pieces.astype(np.int64)
+    inside = (first % nx)[:, None] + lane[None, :] < nx
+    fl = flags.reshape(-1)[np.minimum(first[:, None] + lane[None, :], flags.size - 1)]
+    zero = np.where(inside, fl == 0, True).all(axis=1)
+    assert np.array_equal(plain, zero)
+    # wall pieces are never plain; short pieces of the interior are
+    assert not plain.all()
+    if chunk <= 8:
+        assert plain.any()
 
 
 def test_x_face_flag():
