@@ -262,7 +262,7 @@ def cpu_baseline(budget_s=10.0, single_budget_s=6.0):
     return out
 
 
-def bench_adjoint(n=128, steps=100, reps=20):
+def bench_adjoint(n=128, steps=100, reps=200):
     """Adjoint cost at the differentiation workload's grid (scaled V60, FP32):
     kernel time of one reverse step over one forward step, both from HIP
     events on the launch stream -- with every forward state kept (the
@@ -280,7 +280,7 @@ def bench_adjoint(n=128, steps=100, reps=20):
     prob = AdjointProblem(wl, K, steps=steps, window=20, checkpoint=50)
     ctx, lib = prob.ctx, prob.ctx.lib
     ctx.init_equilibrium()
-    ctx.step(20)  # also warms the forward kernel up
+    ctx.step(600)  # warm-up: the engine clock rises over the first tens of milliseconds
     ev = lambda: torch.cuda.Event(enable_timing=True)
     a, b, c, d = ev(), ev(), ev(), ev()
     torch.cuda.synchronize()
@@ -290,7 +290,7 @@ def bench_adjoint(n=128, steps=100, reps=20):
     state = prob.seg[0]
     state.copy_(ctx.f_current)
     lib.pol_adjoint_reset(ctx.handle, L.stream_handle())
-    for _ in range(3):  # warm-up: the first launch also loads the kernel's code object
+    for _ in range(50):  # warm-up: the first launch also loads the kernel's code object
         L.check(lib.pol_adjoint_step(ctx.handle, L.ptr(state), 1, L.stream_handle()), ctx.handle, "pol_adjoint_step")
     torch.cuda.synchronize()
     c.record()
@@ -346,7 +346,7 @@ def bench_adjoint(n=128, steps=100, reps=20):
     return out
 
 
-def bench_diagnostics(reps=20, host=True):
+def bench_diagnostics(reps=200, host=True):
     """Field diagnostics at 224^3 (V60, FP32): one pol_field_stats call (the
     statistics every reference diagnostic reads: rho / u / flags pass, FP64
     block partials, 43 doubles to the host) timed per call, against the
@@ -363,10 +363,10 @@ def bench_diagnostics(reps=20, host=True):
     wl = WL.v60_config(224, "f32")
     ctx = HipContext(wl.config, wl.geometry.flags)
     ctx.init_equilibrium()
-    ctx.step(20)
+    ctx.step(300)  # also brings the engine clock up to its working value
     ctx.macroscopic(macro=True)
     torch.cuda.synchronize()
-    for _ in range(3):
+    for _ in range(50):
         D.field_statistics(ctx)
     t0 = time.perf_counter()
     for _ in range(reps):
