@@ -651,6 +651,9 @@ def bench_forcing(n=224, reps=100, warmup=10):
     return out
 
 
+# The pour fills the cup as the loop runs, so the warm-up fixes the state that
+# is timed: after 12 steps about 1.03 M water cells (0.350 ms per step), after
+# 100 steps 3.71 M (0.365 ms).  The 12-step state is the one measured.
 def bench_main_loop(n=224, reps=50, warmup=12):
     """The reference's whole main-loop step (main.py:769-839, from step 11
     on) on the 224^3 V60 domain, FP32, every part on the device: clear the
